@@ -1,0 +1,231 @@
+/*
+ * bcmgx.h -- C ABI of the MI355X-native BootCMatchGX single-GPU solve path.
+ *
+ * Every entry point returns an int status (BCMGX_OK == 0, negative on error,
+ * bcmgx_error_string() names it); nothing calls exit().  Handles are opaque.
+ * All state lives in a context (one HIP device, one HIP stream), so distinct
+ * contexts are independent.  Matrices are CSR with int32 row pointers and
+ * column indices and FP64 values; columns must be sorted inside each row and
+ * every row must hold its diagonal entry, as the reference requires
+ * (BCMGX/src/preconditioner/bcmg/matching.cu:155, makeAH).
+ *
+ * Citations are relative to the reference tree (BootCMatchGX v1.1).
+ */
+#ifndef BCMGX_H
+#define BCMGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    BCMGX_OK = 0,
+    BCMGX_ERR_ARG = -1,        /* invalid argument / shape */
+    BCMGX_ERR_HIP = -2,        /* HIP runtime error */
+    BCMGX_ERR_ALLOC = -3,      /* device allocation failed */
+    BCMGX_ERR_IO = -4,         /* file could not be opened or parsed */
+    BCMGX_ERR_PARAM = -5,      /* invalid settings value (INVALID enum) */
+    BCMGX_ERR_UNSUPPORTED = -6,/* option outside the single-GPU path */
+    BCMGX_ERR_NUMERIC = -7,    /* non-finite value / breakdown */
+    BCMGX_ERR_STATE = -8       /* handle used in the wrong state */
+};
+
+/* memory-space flag for pointers crossing the boundary */
+enum { BCMGX_MEM_HOST = 0, BCMGX_MEM_DEVICE = 1 };
+
+/* ---- settings ---------------------------------------------------------
+ * Mirrors `struct params` (src/config/Params.h:238-291) field by field, with
+ * the same defaults, plus extension keys after `rec_res_int`.
+ * Enumerations use the reference's enumerator order.
+ */
+enum { BCMGX_SOLVER_CGHS = 0, BCMGX_SOLVER_FCG = 1, BCMGX_SOLVER_CGS = 2,
+       BCMGX_SOLVER_PIPELINED_CGS = 3, BCMGX_SOLVER_CGS_CUBLAS = 4, BCMGX_SOLVER_INVALID = 5 };
+enum { BCMGX_BOOT_MULTIPLICATIVE = 0, BCMGX_BOOT_SYMMETRIZED_MULTIPLICATIVE = 1,
+       BCMGX_BOOT_ADDITIVE = 2, BCMGX_BOOT_INVALID = 3 };
+enum { BCMGX_MATCH_SUITOR = 0, BCMGX_MATCH_INVALID = 1 };
+enum { BCMGX_CYCLE_V = 0, BCMGX_CYCLE_H = 1, BCMGX_CYCLE_W = 2, BCMGX_CYCLE_VARIABLE_V = 3,
+       BCMGX_CYCLE_INVALID = 4 };
+enum { BCMGX_COARSE_L1_JACOBI = 0, BCMGX_COARSE_INVALID = 1 };
+enum { BCMGX_RELAX_L1_JACOBI = 0, BCMGX_RELAX_INVALID = 1 };
+enum { BCMGX_PREC_NONE = 0, BCMGX_PREC_L1_JACOBI = 1, BCMGX_PREC_BCMG = 2, BCMGX_PREC_AFSAI = 3,
+       BCMGX_PREC_INVALID = 4 };
+/* extension: precision of the AMG hierarchy values and of the V-cycle */
+enum { BCMGX_PRECISION_FP64 = 0, BCMGX_PRECISION_FP32 = 1 };
+
+typedef struct bcmgx_params {
+    int mem_alloc_size;     /* GLOB_MEM_ALLOC_SIZE (unused by this build) */
+    int error;
+    int solver;             /* key "solver", default CGHS */
+    int itnlim;             /* default 2000 */
+    double rtol;            /* default 1e-6 */
+    int dispnorm;           /* default 1 */
+    int preconditioner;     /* key "preconditioner", default NONE */
+    int bootstrap_type;     /* default MULTIPLICATIVE */
+    int max_hrc;            /* default 1 */
+    double conv_ratio;      /* default 0.8 */
+    int matchtype;          /* default SUITOR */
+    int aggrsweeps;         /* default 2 */
+    int aggr_type;          /* default 0 */
+    int max_levels;         /* default 39 */
+    int cycle_type;         /* default V_CYCLE */
+    int coarse_solver;      /* default L1_JACOBI */
+    int relax_type;         /* default L1_JACOBI */
+    int relaxnumber_coarse; /* default 20 */
+    int prerelax_sweeps;    /* default 4 */
+    int postrelax_sweeps;   /* default 4 */
+    int l1jacsweeps;        /* default 4 */
+    int sstep;              /* default 1 */
+    int stop_criterion;     /* default 1 */
+    int ru_res;             /* default 1 */
+    int rec_res_int;        /* default 0 */
+    /* ---- extension keys (no counterpart in the reference) ---- */
+    int max_coarse_size;    /* "max_coarse_size": 0 = reference formula (int)(40*cbrt(N)) */
+    int precision;          /* "precision": FP64 (default) or FP32 hierarchy + V-cycle */
+    int use_graph;          /* "use_graph": capture the FCG iteration in a hipGraph (default 1) */
+    int autotune;           /* "autotune": pick SpMV/SpMM lanes-per-row by timing (default 1) */
+    int check_every;        /* "check_every": FCG iterations between host convergence polls (default 8) */
+} bcmgx_params;
+
+/* Defaults of src/config/Params.h:238-291. */
+int bcmgx_params_default(bcmgx_params* p);
+/* Params::initFromPropertiesFile (src/config/Params.cu:326-404): `key = value`
+ * lines, '#' comments, unknown keys ignored, enum strings as in the reference.
+ * A line without '=' is an error (BCMGX_ERR_IO), as Params.cu:306-309. */
+int bcmgx_params_from_properties(const char* path, bcmgx_params* p);
+/* Set one key (same parsing as a properties line). Unknown key: ignored, returns BCMGX_OK. */
+int bcmgx_params_set(bcmgx_params* p, const char* key, const char* value);
+/* Enum <-> string helpers (Params.cu:16-245). `kind` is one of "solver",
+ * "preconditioner", "cycle_type", "bootstrap_type", "matchtype", "coarse_solver", "relax_type". */
+int bcmgx_enum_from_string(const char* kind, const char* str);
+const char* bcmgx_enum_to_string(const char* kind, int value);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct bcmgx_context_s* bcmgx_context;
+typedef struct bcmgx_matrix_s* bcmgx_matrix;
+typedef struct bcmgx_prec_s* bcmgx_prec;
+
+/* Replaces Handles::init + cudaSetDevice (src/utility/handles.cu, driverSolve.cu:200-205,259). */
+int bcmgx_context_create(int device, bcmgx_context* ctx);
+int bcmgx_context_destroy(bcmgx_context ctx);
+int bcmgx_context_synchronize(bcmgx_context ctx);
+const char* bcmgx_error_string(int status);
+/* Last error message recorded on this context (or globally if ctx == NULL). */
+const char* bcmgx_last_error(bcmgx_context ctx);
+
+/* ---- matrices -----------------------------------------------------------
+ * Replaces CSRm::init + CSRm::copyToDevice (src/datastruct/CSR.h:297,347).
+ * The library copies the arrays into its own device memory.
+ */
+int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, const int* col,
+                     const double* val, int mem, bcmgx_matrix* A);
+int bcmgx_csr_destroy(bcmgx_matrix A);
+int bcmgx_csr_info(bcmgx_matrix A, int* n, int* m, int* nnz);
+/* copy CSR arrays back to host buffers (row: n+1, col/val: nnz) */
+int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val);
+
+/* Host-side problem generators: src/generator/laplacian.cu:119-334 (7-point),
+ * :336-777 (27-point), block ordering internal_index (:14-25).  Output arrays
+ * are malloc'ed by the library and released with bcmgx_free. */
+int bcmgx_gen_laplacian3d(int points, int nx, int ny, int nz, int P, int Q, int R,
+                          int* n, int* nnz, int** row, int** col, double** val);
+/* Extension (no reference counterpart): variable-coefficient 7-point diffusion.
+ * kind 0: face conductivity exp(sigma*g), g ~ N(0,1); kind 1: uniform[lo, hi].
+ * RNG std::mt19937_64(seed). */
+int bcmgx_gen_diffusion7p(int nx, int ny, int nz, int kind, double p1, double p2, uint64_t seed,
+                          int* n, int* nnz, int** row, int** col, double** val);
+/* readMTXDouble + check_and_fix_order (src/datastruct/CSR.cu:1973-2152, :1895-1918). */
+int bcmgx_read_mtx(const char* path, int* n, int* m, int* nnz, int** row, int** col, double** val);
+void bcmgx_free(void* p);
+
+/* ---- preconditioner -----------------------------------------------------
+ * prec_setup (src/preconditioner/prec_setup.h:55): the preconditioner type
+ * comes from params->preconditioner (the reference's cgsprec::ptype).
+ */
+int bcmgx_prec_setup(bcmgx_context ctx, bcmgx_matrix A, const bcmgx_params* p, bcmgx_prec* prec);
+/* prec_apply (src/preconditioner/prec_apply.h:32): u <- B r.  As in the
+ * reference, u is also the initial guess of the V-cycle; FCG zero-fills it.
+ * nrhs columns, column-major with leading dimension ld. */
+int bcmgx_prec_apply(bcmgx_prec prec, const double* r, double* u, int nrhs, int ld, int mem);
+/* prec_finalize (src/preconditioner/prec_finalize.h:29). */
+int bcmgx_prec_finalize(bcmgx_prec prec);
+
+typedef struct bcmgx_prec_info {
+    int ptype;
+    int num_levels;
+    double op_cmplx;        /* AMG::Hierarchy::finalize_cmplx (AMG.cu:143-152) */
+    double op_wcmplx;       /* finalize_wcmplx (AMG.cu:154-163) */
+    double avg_cratio;      /* adaptiveCoarsening (matchingAggregation.cu:437-441) */
+    int max_coarse_size;    /* limit used (reference formula or extension key) */
+    double setup_seconds;   /* device time of the setup (hipEvents) */
+    long long g_nnz;        /* AFSAI: nnz of G */
+} bcmgx_prec_info;
+int bcmgx_prec_get_info(bcmgx_prec prec, bcmgx_prec_info* info);
+/* level sizes: n and nnz of A_level */
+int bcmgx_prec_level_size(bcmgx_prec prec, int level, int* n, int* nnz);
+/* copy A_level to host (row: n+1, col/val: nnz) */
+int bcmgx_prec_level_matrix(bcmgx_prec prec, int level, int* row, int* col, double* val);
+/* prolongator of level -> level+1: one entry per fine row: coarse index and value */
+int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* pval);
+/* AFSAI factor G (lower triangular, row-sorted) */
+int bcmgx_prec_afsai_factor(bcmgx_prec prec, int* n, int* nnz, int* row, int* col, double* val);
+
+/* ---- solver ---------------------------------------------------------------
+ * solve (src/solver/solve.h:33) -> solve_fcg (src/solver/fcg/FCG.h:37).
+ * x is in/out (initial guess, then solution), as the reference's x0.
+ * nrhs >= 1 columns (column-major, leading dimension ld) share one
+ * preconditioner; each column keeps its own FCG scalars and stops on its own
+ * criterion ||r_k|| <= rtol ||r_0|| (FCG.cu:93,156).
+ */
+typedef struct bcmgx_solver_out {
+    int retv;               /* 0 converged, -1 itnlim reached (FCG.cu:160-162); worst column */
+    int niter;              /* reference counting (FCG.cu:164); max over columns */
+    double exit_res;        /* ||r|| at exit (FCG.cu:165); max over columns */
+    double rel_res;         /* exit_res / ||r_0||; max over columns */
+    double del0;            /* ||r_0|| of column 0 */
+    double solve_seconds;   /* device time of the solve */
+    int full_n;
+    int local_n;
+} bcmgx_solver_out;
+int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_params* p,
+                const double* b, double* x, int nrhs, int ld, int mem, bcmgx_solver_out* out);
+/* per-column results of the last bcmgx_solve on this preconditioner */
+int bcmgx_solve_columns(bcmgx_prec prec, int nrhs, int* niter, double* exit_res, double* rel_res);
+
+/* info file of src/solver/SolverOut.cu:9-97 with AMG::Hierarchy::printInfo (AMG.cu:167-180) */
+int bcmgx_dump_info(const char* path, const bcmgx_params* p, bcmgx_prec prec, const bcmgx_solver_out* out);
+
+/* ---- kernel-level entry points (parity tests and micro-benchmarks) --------
+ * y = alpha*A*x + beta*y, op as _CSR_vector_mul_mini_warp<OP> (CSR.cu:452-492):
+ * op 1: y = A x; op 2: y = y - A x; op 0: y = alpha A x + beta y.
+ * lanes: lanes per row (0 = autotuned choice). */
+int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, double alpha,
+               double beta, int op, int lanes, int mem);
+/* C = A*B (replaces nsparse spgemm_csrseg_kernel_hash, kernel_spgemm_hash_d.cu:1196-1247),
+ * sorted columns; each C_ij summed over k in ascending order. */
+int bcmgx_spgemm(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix B, bcmgx_matrix* C);
+/* C = A^T (CSRm::Transpose_local, CSR.cu:1752-1822) */
+int bcmgx_transpose(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix* T);
+/* Suitor matching of suitor() (matching.cu:253-288): M[i] = partner or -1. */
+int bcmgx_matching(bcmgx_context ctx, bcmgx_matrix A, const double* w, int* M);
+/* makeP_GPU (matchingPairAggregation.cu:109-170), canonical singleton order:
+ * agg[i] coarse index, pval[i] value; returns the coarse size in *nc. */
+int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double* w, int* agg,
+                           double* pval, int* nc);
+/* Sustained device-to-device copy bandwidth (GB/s) over `bytes` bytes. */
+int bcmgx_stream_copy_gbs(bcmgx_context ctx, size_t bytes, int reps, double* gbs);
+
+/* ---- adjoint support for the differentiable solve -------------------------
+ * Extension (no reference counterpart): g[k] = -sum_c lambda[row(k), c] x[col(k), c]
+ * for every stored entry k of A (the sparsity-restricted outer product that
+ * is dL/dA_ij for x = A^{-1} b). lambda, x: n x nrhs column-major. */
+int bcmgx_sparse_outer_grad(bcmgx_context ctx, bcmgx_matrix A, const double* lambda,
+                            const double* x, int nrhs, double* g, int mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

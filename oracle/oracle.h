@@ -1,0 +1,58 @@
+/*
+ * oracle.h -- C ABI of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++17 (+ optional OpenMP)
+ * restatement of BootCMatchGX's single-GPU solve path, used as the checker
+ * for the HIP product and as the timed CPU baseline.  Nothing in the product
+ * (bootcmatchgx_amd/) may link or call it.
+ */
+#ifndef BCMGX_ORACLE_H
+#define BCMGX_ORACLE_H
+
+#include "../include/bcmgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ocsr ocsr;
+typedef struct oprec oprec;
+
+ocsr* or_csr_new(int n, int m, int nnz, const int* row, const int* col, const double* val);
+void or_csr_free(ocsr* A);
+void or_csr_dims(const ocsr* A, int* n, int* m, int* nnz);
+void or_csr_copy(const ocsr* A, int* row, int* col, double* val);
+
+ocsr* or_gen_laplacian3d(int points, int nx, int ny, int nz, int P, int Q, int R);
+ocsr* or_gen_diffusion7p(int nx, int ny, int nz, int kind, double p1, double p2, unsigned long long seed);
+ocsr* or_read_mtx(const char* path);
+
+void or_spmv(const ocsr* A, const double* x, double* y, double alpha, double beta, int op);
+ocsr* or_spgemm(const ocsr* A, const ocsr* B);
+ocsr* or_transpose(const ocsr* A);
+
+/* matching weights W (log |AH|, symmetrised) laid out on A's off-diagonal pattern */
+int or_matching_weights(const ocsr* A, const double* w, int* Wrow, int* Wcol, double* Wval);
+void or_suitor(const ocsr* A, const double* w, int* M);
+void or_greedy_matching(const ocsr* A, const double* w, int* M);
+int or_make_p(int n, const int* M, const double* w, int* agg, double* pval);
+
+oprec* or_prec_setup(const ocsr* A, const bcmgx_params* p);
+void or_prec_free(oprec* pr);
+int or_prec_levels(const oprec* pr);
+const ocsr* or_prec_level_A(const oprec* pr, int level);
+int or_prec_level_P(const oprec* pr, int level, int* agg, double* pval);
+void or_prec_hierarchy_stats(const oprec* pr, double* op_cmplx, double* op_wcmplx, double* avg_cratio, int* maxcoarse);
+const ocsr* or_prec_afsai_G(const oprec* pr);
+void or_prec_apply(const oprec* pr, const double* r, double* u);
+
+/* FCG (FCG.cu:27-175).  x in/out.  Returns retv. */
+int or_fcg(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
+           int* niter, double* exit_res, double* del0);
+
+int or_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
