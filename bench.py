@@ -1,0 +1,204 @@
+"""Benchmark of the single-GPU AMG-FCG path on MI355X.
+
+Workload (one step): the 27-point Laplacian on a 128^3 grid (2,097,152 rows,
+55,742,968 nonzeros) with 32 right-hand sides drawn N(0,1) (numpy PCG64,
+seed 3), solved to relative residual 1e-8 by FCG preconditioned with one
+BCMG V-cycle (3 pairwise matching sweeps per level, 1 pre / 1 post l1-Jacobi
+sweep, 20 coarsest sweeps), FP32 hierarchy and V-cycle under the FP64 FCG,
+all 32 columns in one batched solve.  A step is AMG setup + batched solve
+(time-to-solution); inputs are resident in HBM when the timed region starts.
+
+Reported beside it: the plain FP64 path solving the 32 right-hand sides one at
+a time (one setup), the FP64 SpMV of the fine matrix against the HBM roofline
+(HIP-event timed on the library stream), a STREAM-copy ceiling, and the CPU
+oracle's FP64 setup + solve on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AMG-FCG time-to-1e-8 (s) + SpMV HBM GB/s, 27p 128^3; mixed/batched vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_fp64_bytes(n, nnz):
+    # 12 nnz (val + col) + 4 (n + 1) (row pointers) + 8 n (x) + 8 n (y)
+    return 12 * nnz + 4 * (n + 1) + 16 * n
+
+
+def cpu_baseline(size, nrhs, rtol):
+    """The oracle (test infrastructure) timed on the host: FP64 setup + one solve."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    from bootcmatchgx_amd import abi
+
+    threads = O.lib().or_num_threads()
+    A = O.laplacian3d(27, size, size, size)
+    n = A.dims()[0]
+    b = np.random.default_rng(3).normal(size=n)
+    p = abi.reference_defaults()
+    p.solver = abi.SOLVER["FCG"]
+    p.preconditioner = abi.PREC["BCMG"]
+    p.rtol = rtol
+    p.prerelax_sweeps = 1
+    p.postrelax_sweeps = 1
+    p.dispnorm = 0
+    t0 = time.perf_counter()
+    P = O.Prec(A, p)
+    t1 = time.perf_counter()
+    _, info = O.fcg(A, P, p, b)
+    t2 = time.perf_counter()
+    setup, solve = t1 - t0, t2 - t1
+    return {
+        "value": setup + nrhs * solve,
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"27p {size}^3 FP64 BCMG setup ({setup:.2f} s) + one FCG solve of column 0 "
+                  f"({solve:.2f} s, {info['niter']} iterations); value = setup + {nrhs} x solve",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 one-at-a-time comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU row-block path is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import bootcmatchgx_amd as bx
+
+    N, k, rtol = args.size, args.nrhs, 1e-8
+    ctx = bx.Context(0)
+    n, row, col, val = bx.laplacian3d(27, N, N, N)
+    nnz = len(col)
+    A = bx.CSR(ctx, n, n, row, col, val)
+    del row, col, val
+    Bh = np.asfortranarray(np.random.default_rng(3).normal(size=(n, k)))
+    B = bx.DeviceArray(ctx, (n, k))
+    B.upload(Bh.ravel(order="F"))
+    X = bx.DeviceArray(ctx, (n, k))
+    Z = bx.DeviceArray(ctx, (n, k))
+    Z.upload(np.zeros(n * k))
+
+    common = dict(solver="FCG", preconditioner="BCMG", rtol=rtol, prerelax_sweeps=1, postrelax_sweeps=1, dispnorm=0)
+    p32 = bx.Params(precision="FP32", **common)
+    p64 = bx.Params(precision="FP64", **common)
+
+    def step():
+        X.copy_from(Z)
+        t0 = time.perf_counter()
+        P = bx.prec_setup(ctx, A, p32)
+        t1 = time.perf_counter()
+        out = bx.solve_device(ctx, A, B.ptr, X.ptr, k, p32, P)
+        t2 = time.perf_counter()
+        P.finalize()
+        return t2 - t0, t1 - t0, t2 - t1, out
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    times, setups, solves, outs = [], [], [], []
+    for _ in range(args.steps):
+        t, ts, tv, out = step()
+        times.append(t)
+        setups.append(ts)
+        solves.append(tv)
+        outs.append(out)
+    ctx.synchronize()
+    X32 = X.download().reshape((n, k), order="F")
+    value = float(np.mean(times))
+    last = outs[-1]
+
+    # plain FP64 path, one right-hand side at a time (one setup)
+    fp64 = None
+    if not args.no_fp64:
+        X.copy_from(Z)
+        t0 = time.perf_counter()
+        P = bx.prec_setup(ctx, A, p64)
+        t1 = time.perf_counter()
+        it64 = []
+        for j in range(k):
+            o = bx.solve_device(ctx, A, B.at(j * n), X.at(j * n), 1, p64, P)
+            it64.append(o["niter"])
+        t2 = time.perf_counter()
+        P.finalize()
+        X64 = X.download().reshape((n, k), order="F")
+        fp64 = {"time_s": t2 - t0, "setup_s": t1 - t0, "solve_s": t2 - t1, "niter_max": max(it64),
+                "err_mixed_vs_fp64": float(np.linalg.norm(X32 - X64) / np.linalg.norm(X64))}
+
+    # SpMV FP64 of the fine matrix against the HBM roofline (HIP events on the library stream)
+    spmv_ms = A.time_spmv(precision=0, k=1, reps=50)
+    spmv_bytes = spmv_fp64_bytes(n, nnz)
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    spmm32_ms = A.time_spmv(precision=1, k=k, reps=10)
+    copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
+
+    cpu = None if args.no_cpu else cpu_baseline(N, k, rtol)
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": value * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 hierarchy / f64 krylov",
+        "data": "synthetic",
+        "config": {
+            "workload": f"27-point Laplacian {N}^3, {k} RHS N(0,1) seed 3, FCG+BCMG rtol 1e-8, "
+                        "V-cycle 1/1 l1-Jacobi, FP32 hierarchy, batched",
+            "n": n, "nnz": nnz, "nrhs": k,
+        },
+        "setup_s": float(np.mean(setups)),
+        "solve_s": float(np.mean(solves)),
+        "niter": last["niter"],
+        "final_rel_res": last["rel_res"],
+        "fp64_one_at_a_time": fp64,
+        "speedup_vs_fp64_one_at_a_time": (fp64["time_s"] / value) if fp64 else None,
+        "spmv_fp64_gbs": spmv_gbs,
+        "spmm_fp32_k32_ms": spmm32_ms,
+        "stream_copy_gbs": copy_gbs,
+        "roofline": {
+            "kernel": "k_spmv1<double,double,L,AX> (fine 27-point matrix, FP64, one vector)",
+            "bound": "hbm",
+            "achieved": spmv_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": spmv_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": spmv_bytes,
+            "avg_launch_ms": spmv_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+    A.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

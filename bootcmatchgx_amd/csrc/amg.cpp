@@ -1,0 +1,578 @@
+// amg.cpp -- BCMG hierarchy build, preconditioner application and the FCG
+// driver, all on one HIP stream of the context.
+//
+// Hierarchy: adaptiveCoarsening / matchingAggregation
+// (src/preconditioner/bcmg/matchingAggregation.cu:106-466): per level,
+// aggrsweeps+1 pairwise steps, each suitor matching -> P_i -> A_{i+1} =
+// R_i (A_i P_i) and w_{i+1} = R_i w_i, composing P <- P P_i.
+// Cycle: GAMG_cycle (GAMG_cycle.cu:58-236) with l1-Jacobi smoothing; the
+// first sweep from a zero guess is the diagonal scaling f/M (bit-identical to
+// the reference's full sweep on zero), and the ping-pong buffer holding a
+// level's result is tracked instead of copied back.
+// FCG: flexibileConjugateGradients_v3 (src/solver/fcg/FCG.cu:27-175); one
+// iteration pair is captured in a hipGraph and replayed, the host polls the
+// device stop flag every `check_every` iterations.
+#include "amg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <type_traits>
+
+namespace bcmgx {
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------ autotune ----
+int tune_lanes(Context* c, const DCsr& A, bool fp32) {
+    if (A.n < 65536) return default_lanes(A.nnz, A.n);
+    std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d");
+    auto it = c->tuned_lanes.find(sig);
+    if (it != c->tuned_lanes.end()) return it->second;
+    int base = default_lanes(A.nnz, A.n);
+    int cands[3] = {std::max(1, base / 2), base, std::min(64, base * 2)};
+    hipEvent_t e0, e1;
+    BCMGX_HIP(hipEventCreate(&e0));
+    BCMGX_HIP(hipEventCreate(&e1));
+    int best = base;
+    float bestms = 1e30f;
+    if (fp32) {
+        DBuf<float> x(c, A.n), y(c, A.n);
+        fill<float>(c, x.get(), A.n, 1.0f);
+        for (int L : cands) {
+            SpmvArgs a;
+            a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.valf.get(); a.nnz = A.nnz;
+            a.x = x.get(); a.y = y.get(); a.lanes = L;
+            spmv<float, float>(c, SpOp::AX, a);
+            BCMGX_HIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 5; r++) spmv<float, float>(c, SpOp::AX, a);
+            BCMGX_HIP(hipEventRecord(e1, c->stream));
+            BCMGX_HIP(hipEventSynchronize(e1));
+            float ms;
+            BCMGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < bestms) { bestms = ms; best = L; }
+        }
+    } else {
+        DBuf<double> x(c, A.n), y(c, A.n);
+        fill<double>(c, x.get(), A.n, 1.0);
+        for (int L : cands) {
+            SpmvArgs a;
+            a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+            a.x = x.get(); a.y = y.get(); a.lanes = L;
+            spmv<double, double>(c, SpOp::AX, a);
+            BCMGX_HIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 5; r++) spmv<double, double>(c, SpOp::AX, a);
+            BCMGX_HIP(hipEventRecord(e1, c->stream));
+            BCMGX_HIP(hipEventSynchronize(e1));
+            float ms;
+            BCMGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < bestms) { bestms = ms; best = L; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tuned_lanes[sig] = best;
+    return best;
+}
+
+// ----------------------------------------------------------- hierarchy ----
+// setMaxCoarseSize (AMG.cu:238-241), truncation included
+static int reference_max_coarse_size(long long n) { return (int)(40 * std::pow((double)n, (double)1. / 3.)); }
+
+static void check_diagonals(Context* c, const DCsr& A) {
+    DBuf<int> dind(c, std::max(A.n, 1));
+    diag_index(c, A.n, A.row.get(), A.col.get(), dind.get());
+    std::vector<int> h(A.n);
+    c->d2h(h.data(), dind.get(), A.n);
+    for (int i = 0; i < A.n; i++)
+        BCMGX_CHECK(h[i] >= 0, BCMGX_ERR_ARG, "row " + std::to_string(i) + " has no diagonal entry (required by the matching)");
+}
+
+static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H) {
+    BCMGX_CHECK(A0.n == A0.m, BCMGX_ERR_ARG, "the matrix must be square");
+    check_diagonals(c, A0);
+    int maxcs = p.max_coarse_size > 0 ? p.max_coarse_size : reference_max_coarse_size(A0.n);
+    H.maxcs = maxcs;
+    int sweeps = p.aggrsweeps + 1;
+    H.L.clear();
+    H.L.emplace_back(new Level());
+    H.L[0]->Aref = &A0;
+    H.L[0]->M.reset(c, A0.n);
+    abs_row_sum(c, A0.n, A0.row.get(), A0.val.get(), H.L[0]->M.get());
+
+    DBuf<double> w(c, A0.n);
+    fill<double>(c, w.get(), A0.n, 1.0);
+    int ftcoarse = 1;
+    double avc = 0.0;
+    for (int level = 1; level < p.max_levels;) {
+        Level& prev = *H.L[level - 1];
+        const DCsr* Ai = &prev.mat();
+        DCsr Aown;  // intermediate sweep operator
+        DBuf<double> wi = std::move(w);
+        DBuf<int> agg;
+        DBuf<double> pv;
+        DCsr Afinal;
+        for (int s = 0; s < sweeps; s++) {
+            int n = Ai->n;
+            DBuf<int> M(c, n), agg_i(c, n);
+            DBuf<double> pval_i(c, n);
+            suitor_match(c, *Ai, wi.get(), M.get());
+            int nc = make_prolongator(c, n, M.get(), wi.get(), agg_i.get(), pval_i.get());
+            BCMGX_CHECK(nc > 0, BCMGX_ERR_NUMERIC, "coarsening produced an empty level");
+            DCsr Pi, Ri, AP, Anew;
+            prolongator_csr(c, n, nc, agg_i.get(), pval_i.get(), Pi);
+            transpose_prolongator(c, n, nc, agg_i.get(), pval_i.get(), Ri);
+            spgemm(c, *Ai, Pi, AP);
+            spgemm(c, Ri, AP, Anew);
+            DBuf<double> wn(c, nc);
+            {
+                SpmvArgs a;  // w_{i+1} = R_i w_i, one lane per row (sequential sums)
+                a.n = nc; a.row = Ri.row.get(); a.col = Ri.col.get(); a.val = Ri.val.get(); a.nnz = Ri.nnz;
+                a.x = wi.get(); a.y = wn.get(); a.lanes = 1;
+                spmv<double, double>(c, SpOp::AX, a);
+            }
+            double size_pre = n, size_c = nc;
+            if (size_pre / size_c <= 1.2) ftcoarse = 100;  // COARSERATIO_THRSLD, FTCOARSE_INC
+            if (s == 0) {
+                agg = std::move(agg_i);
+                pv = std::move(pval_i);
+            } else {
+                compose_prolongators(c, (int)agg.size(), agg.get(), pv.get(), agg_i.get(), pval_i.get());
+            }
+            Aown = std::move(Anew);
+            Ai = &Aown;
+            wi = std::move(wn);
+            if (size_c <= (double)ftcoarse * maxcs) break;
+        }
+        w = std::move(wi);
+        std::unique_ptr<Level> nl(new Level());
+        nl->A = std::move(Aown);
+        nl->M.reset(c, nl->A.n);
+        abs_row_sum(c, nl->A.n, nl->A.row.get(), nl->A.val.get(), nl->M.get());
+        int nc = nl->A.n;
+        prev.agg = std::move(agg);
+        prev.pval = std::move(pv);
+        transpose_prolongator(c, prev.mat().n, nc, prev.agg.get(), prev.pval.get(), prev.R);
+        avc += (double)prev.mat().n / (double)nc;
+        H.L.push_back(std::move(nl));
+        level++;
+        if ((double)nc <= (double)ftcoarse * maxcs) break;
+    }
+    int nlv = H.levels();
+    double nnz0 = (double)H.L[0]->mat().nnz, cx = 0, cw = 0;
+    for (int i = 0; i < nlv; i++) {
+        cx += H.L[i]->mat().nnz;
+        cw += std::pow(2, i) * H.L[i]->mat().nnz;
+    }
+    H.op_cmplx = cx / nnz0;
+    H.op_wcmplx = cw / nnz0;
+    H.avg_cratio = nlv > 1 ? avc / (nlv - 1) : 1;
+}
+
+static void make_fp32_copies(Context* c, Hierarchy& H) {
+    for (auto& lp : H.L) {
+        Level& l = *lp;
+        const DCsr& A = l.mat();
+        DCsr& Am = const_cast<DCsr&>(A);  // the FP32 copy is a cache of the same operator
+        if (!Am.valf) {
+            Am.valf.reset(c, std::max(A.nnz, 1));
+            convert<double, float>(c, A.val.get(), Am.valf.get(), A.nnz);
+        }
+        l.Mf.reset(c, std::max(A.n, 1));
+        convert<double, float>(c, l.M.get(), l.Mf.get(), A.n);
+        if (l.pval) {
+            l.pvalf.reset(c, l.pval.size());
+            convert<double, float>(c, l.pval.get(), l.pvalf.get(), (long long)l.pval.size());
+            l.R.valf.reset(c, std::max(l.R.nnz, 1));
+            convert<double, float>(c, l.R.val.get(), l.R.valf.get(), l.R.nnz);
+        }
+    }
+}
+
+// ------------------------------------------------------------ prec ----
+Prec::~Prec() {}
+
+void Prec::ensure_work(int k) {
+    if (k <= kcap) return;
+    kcap = k;
+    if (ptype == BCMGX_PREC_BCMG) {
+        int nl = H.levels();
+        if (p.precision == BCMGX_PRECISION_FP32) {
+            wf.clear();
+            wf.resize(nl);
+            for (int l = 0; l < nl; l++) {
+                size_t sz = (size_t)std::max(H.L[l]->mat().n, 1) * k;
+                wf[l].F.reset(ctx, sz); wf[l].X.reset(ctx, sz); wf[l].X2.reset(ctx, sz); wf[l].Res.reset(ctx, sz);
+            }
+            size_t cs = (size_t)std::max(H.L[nl - 1]->mat().n, 1) * k;
+            cd_F.reset(ctx, cs); cd_X.reset(ctx, cs); cd_X2.reset(ctx, cs);
+        } else {
+            wd.clear();
+            wd.resize(nl);
+            for (int l = 0; l < nl; l++) {
+                size_t sz = (size_t)std::max(H.L[l]->mat().n, 1) * k;
+                if (l > 0) wd[l].F.reset(ctx, sz);
+                wd[l].X.reset(ctx, sz); wd[l].X2.reset(ctx, sz); wd[l].Res.reset(ctx, sz);
+            }
+        }
+    }
+    tmp.reset(ctx, (size_t)std::max(A->n, 1) * k);
+}
+
+template <typename V>
+static const V* mat_val(const DCsr& A) {
+    if constexpr (std::is_same<V, float>::value) return A.valf.get();
+    else return A.val.get();
+}
+
+// k sweeps of l1-Jacobi; (cur, oth) is the ping-pong pair, cur holds the
+// guess (ignored when xzero).  Returns the buffer holding the result.
+template <typename V>
+static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const V* F, V* cur, V* oth, int s, bool xzero,
+                        int k, const int* stop) {
+    int done = 0;
+    if (s <= 0) {
+        if (xzero) fill<V>(c, cur, (long long)A.n * k, (V)0);
+        return cur;
+    }
+    if (xzero) {
+        jacobi_from_zero<V>(c, A.n, k, F, M, cur, stop);
+        done = 1;
+    }
+    for (; done < s; done++) {
+        SpmvArgs a;
+        a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
+        a.x = cur; a.y = oth; a.b = F; a.D = M; a.k = k; a.lanes = lanes; a.stop = stop;
+        spmv<V, V>(c, SpOp::JACOBI, a);
+        std::swap(cur, oth);
+    }
+    return cur;
+}
+
+template <typename V>
+struct CycleRun {
+    Context* c;
+    Prec* pr;
+    std::vector<LevelWork<V>>* W;
+    int k;
+    const int* stop;
+};
+
+template <typename V>
+static const V* lvl_M(const Level& l) {
+    if constexpr (std::is_same<V, float>::value) return l.Mf.get();
+    else return l.M.get();
+}
+template <typename V>
+static const V* lvl_pval(const Level& l) {
+    if constexpr (std::is_same<V, float>::value) return l.pvalf.get();
+    else return l.pval.get();
+}
+
+// GAMG_cycle (GAMG_cycle.cu:58-236), 0-based level l.  Xin: initial guess
+// buffer (one of W[l].X / W[l].X2) or nullptr for zero.  Returns the buffer
+// holding the level's result.
+template <typename V>
+static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin) {
+    Context* c = R.c;
+    Prec* pr = R.pr;
+    const bcmgx_params& p = pr->p;
+    Hierarchy& H = pr->H;
+    int nl = H.levels();
+    Level& lv = *H.L[l];
+    const DCsr& A = lv.mat();
+    LevelWork<V>& w = (*R.W)[l];
+    V* cur = Xin ? Xin : w.X.get();
+    V* oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
+    bool xzero = Xin == nullptr;
+    int k = R.k;
+    if (l == nl - 1) {
+        if constexpr (std::is_same<V, float>::value) {
+            // the coarsest-level sweeps run in FP64 on the FP64 operator
+            long long nk = (long long)A.n * k;
+            convert_stop<float, double>(c, F, pr->cd_F.get(), nk, R.stop);
+            if (!xzero) convert_stop<float, double>(c, cur, pr->cd_X.get(), nk, R.stop);
+            double* res = jacobi_sweeps<double>(c, A, lv.lanesA, lv.M.get(), pr->cd_F.get(), pr->cd_X.get(), pr->cd_X2.get(),
+                                                p.relaxnumber_coarse, xzero, k, R.stop);
+            convert_stop<double, float>(c, res, cur, nk, R.stop);
+            return cur;
+        } else {
+            return jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.relaxnumber_coarse, xzero, k, R.stop);
+        }
+    }
+    int coeff = p.cycle_type == BCMGX_CYCLE_VARIABLE_V ? (1 << l) : 1;
+    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff, xzero, k, R.stop);
+    oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
+    // Res = F - A x ; F_{l+1} = R Res
+    {
+        SpmvArgs a;
+        a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
+        a.x = cur; a.y = w.Res.get(); a.b = F; a.k = k; a.lanes = lv.lanesA; a.stop = R.stop;
+        spmv<V, V>(c, SpOp::RES, a);
+    }
+    LevelWork<V>& wc = (*R.W)[l + 1];
+    {
+        SpmvArgs a;
+        a.n = lv.R.n; a.row = lv.R.row.get(); a.col = lv.R.col.get(); a.val = mat_val<V>(lv.R); a.nnz = lv.R.nnz;
+        a.x = w.Res.get(); a.y = wc.F.get(); a.k = k; a.lanes = lv.lanesR; a.stop = R.stop;
+        spmv<V, V>(c, SpOp::AX, a);
+    }
+    V* xc = nullptr;
+    if (nl > 2 || p.relaxnumber_coarse > 0) {
+        int gs = 1;  // setGridSweeps (AMG.cu:306-331)
+        if (p.cycle_type == BCMGX_CYCLE_W && l < nl - 2) gs = 2;
+        if (p.cycle_type == BCMGX_CYCLE_H && (l % 2) == 0) gs = 2;
+        for (int i = 1; i <= gs; i++) {
+            xc = gamg_cycle<V>(R, l + 1, wc.F.get(), xc);
+            if (l + 1 == nl - 1) break;
+        }
+    }
+    if (xc) prolong_add<V>(c, A.n, k, lv.agg.get(), lvl_pval<V>(lv), xc, cur, R.stop);
+    oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
+    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.postrelax_sweeps * coeff, false, k, R.stop);
+    return cur;
+}
+
+void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop) {
+    ensure_work(k);
+    Context* c = ctx;
+    long long nk = (long long)A->n * k;
+    switch (ptype) {
+    case BCMGX_PREC_NONE:
+        copy_d2d(c, u, r, sizeof(double) * nk);
+        break;
+    case BCMGX_PREC_L1_JACOBI: {  // l1jacobi_apply (l1jacobi.cu:513-523)
+        double* res = jacobi_sweeps<double>(c, *A, H.L.empty() ? 0 : 0, pl1j.get(), r, u, tmp.get(), p.l1jacsweeps, x_zero, k, stop);
+        if (res != u) copy_d2d(c, u, res, sizeof(double) * nk);
+        break;
+    }
+    case BCMGX_PREC_AFSAI: {  // afsai_apply (afsai.cu:1092-1103): u = G^T (G r)
+        SpmvArgs a;
+        a.n = G.n; a.row = G.row.get(); a.col = G.col.get(); a.val = G.val.get(); a.nnz = G.nnz;
+        a.x = r; a.y = tmp.get(); a.k = k; a.lanes = lanesG; a.stop = stop;
+        spmv<double, double>(c, SpOp::AX, a);
+        SpmvArgs b;
+        b.n = GT.n; b.row = GT.row.get(); b.col = GT.col.get(); b.val = GT.val.get(); b.nnz = GT.nnz;
+        b.x = tmp.get(); b.y = u; b.k = k; b.lanes = lanesGT; b.stop = stop;
+        spmv<double, double>(c, SpOp::AX, b);
+        break;
+    }
+    case BCMGX_PREC_BCMG: {  // bcmg_apply (bcmg.cu:51-90)
+        if (p.precision == BCMGX_PRECISION_FP32) {
+            CycleRun<float> R{c, this, &wf, k, stop};
+            convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
+            float* xin = nullptr;
+            if (!x_zero) {
+                convert_stop<double, float>(c, u, wf[0].X.get(), nk, stop);
+                xin = wf[0].X.get();
+            }
+            float* res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin);
+            convert_stop<float, double>(c, res, u, nk, stop);
+        } else {
+            CycleRun<double> R{c, this, &wd, k, stop};
+            double* xin = nullptr;
+            if (!x_zero) {
+                copy_d2d(c, wd[0].X.get(), u, sizeof(double) * nk);
+                xin = wd[0].X.get();
+            }
+            double* res = gamg_cycle<double>(R, 0, r, xin);
+            convert_stop<double, double>(c, res, u, nk, stop);
+        }
+        break;
+    }
+    default:
+        throw Error(BCMGX_ERR_PARAM, "unhandled preconditioner type");
+    }
+}
+
+std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& p) {
+    double t0 = now_s();
+    std::unique_ptr<Prec> pr(new Prec());
+    pr->ctx = c;
+    pr->A = &A;
+    pr->p = p;
+    pr->ptype = p.preconditioner;
+    switch (p.preconditioner) {
+    case BCMGX_PREC_NONE:
+        break;
+    case BCMGX_PREC_L1_JACOBI:
+        pr->pl1j.reset(c, std::max(A.n, 1));
+        l1_diag(c, A.n, A.row.get(), A.col.get(), A.val.get(), pr->pl1j.get());
+        break;
+    case BCMGX_PREC_AFSAI:
+        afsai_build(c, A, pr->G, pr->GT);
+        pr->lanesG = default_lanes(pr->G.nnz, pr->G.n);
+        pr->lanesGT = default_lanes(pr->GT.nnz, pr->GT.n);
+        break;
+    case BCMGX_PREC_BCMG: {
+        BCMGX_CHECK(p.max_hrc == 1, BCMGX_ERR_UNSUPPORTED, "bootstrap with more than one hierarchy is not supported (bootstrap.cu:37-38)");
+        BCMGX_CHECK(p.aggr_type == 0, BCMGX_ERR_UNSUPPORTED, "aggr_type != 0 is not supported");
+        BCMGX_CHECK(p.bootstrap_type == BCMGX_BOOT_MULTIPLICATIVE, BCMGX_ERR_UNSUPPORTED,
+                    "only the MULTIPLICATIVE composition is applied (bcmg.cu:60-87)");
+        BCMGX_CHECK(p.cycle_type >= 0 && p.cycle_type < BCMGX_CYCLE_INVALID, BCMGX_ERR_PARAM, "invalid cycle_type");
+        build_hierarchy(c, A, p, pr->H);
+        bool f32 = p.precision == BCMGX_PRECISION_FP32;
+        if (f32) make_fp32_copies(c, pr->H);
+        for (auto& lp : pr->H.L) {
+            Level& l = *lp;
+            const DCsr& M = l.mat();
+            l.lanesA = p.autotune ? tune_lanes(c, M, f32) : default_lanes(M.nnz, M.n);
+            l.lanesR = l.R.n > 0 ? default_lanes(l.R.nnz, l.R.n) : 1;
+        }
+        break;
+    }
+    default:
+        throw Error(BCMGX_ERR_PARAM, "invalid preconditioner type");
+    }
+    pr->ensure_work(1);
+    c->sync();
+    pr->setup_seconds = now_s() - t0;
+    return pr;
+}
+
+// ---------------------------------------------------------------- FCG ----
+struct SolveWork {
+    int k = 0, n = 0;
+    DBuf<double> B, X, r, v, w, d, q;
+    DBuf<double> partials;
+    DBuf<FcgState> st;
+    DBuf<int> flags;  // [0] all_done, [1] columns done
+    FcgBufs fb;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int lanesA = 0;
+    ~SolveWork() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, int idx) {
+    double* P0 = idx == 0 ? s.v.get() : s.d.get();
+    double* W0 = idx == 0 ? s.w.get() : s.q.get();
+    double* P1 = idx == 0 ? s.d.get() : s.v.get();
+    double* W1 = idx == 0 ? s.q.get() : s.w.get();
+    const int* stop = s.flags.get();
+    prec->apply(s.r.get(), P0, s.k, true, stop);  // P0 zero-filled in the reference (FCG.cu:106,119)
+    SpmvArgs a;
+    a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+    a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop;
+    spmv<double, double>(c, SpOp::AX, a);
+    fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1);
+    fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get());
+    fcg_finish_iteration(c, s.fb);
+}
+
+SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k) {
+    BCMGX_CHECK(k >= 1 && k <= 64, BCMGX_ERR_ARG, "nrhs must be in [1, 64]");
+    BCMGX_CHECK(p.solver == BCMGX_SOLVER_FCG, BCMGX_ERR_UNSUPPORTED, "only the FCG solver is on the single-GPU path");
+    double t0 = now_s();
+    int n = A.n;
+    SolveWork* sp = nullptr;
+    for (auto& w : prec->solves)
+        if (w->k == k && w->n == n) sp = w.get();
+    if (!sp) {
+        prec->solves.emplace_back(new SolveWork());
+        sp = prec->solves.back().get();
+        SolveWork& s = *sp;
+        s.k = k;
+        s.n = n;
+        size_t nk = (size_t)std::max(n, 1) * k;
+        s.B.reset(c, nk); s.X.reset(c, nk); s.r.reset(c, nk); s.v.reset(c, nk);
+        s.w.reset(c, nk); s.d.reset(c, nk); s.q.reset(c, nk);
+        s.fb.n = n;
+        s.fb.k = k;
+        s.fb.nblk = fcg_partial_blocks(n);
+        s.partials.reset(c, (size_t)s.fb.nblk * k * 3);
+        s.st.reset(c, k);
+        s.flags.reset(c, 2);
+        s.fb.partials = s.partials.get();
+        s.fb.st = s.st.get();
+        s.fb.all_done = s.flags.get();
+        s.lanesA = p.autotune ? tune_lanes(c, A, false) : default_lanes(A.nnz, A.n);
+    }
+    SolveWork& s = *sp;
+    s.fb.rtol = p.rtol;
+    s.fb.itnlim = p.itnlim;
+    prec->ensure_work(k);
+    long long nk = (long long)n * k;
+    copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
+    copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
+    fill<int>(c, s.flags.get(), 2, 0);
+    // ---- initial step (FCG.cu:44-99)
+    {
+        SpmvArgs a;
+        a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        a.x = s.X.get(); a.y = s.r.get(); a.b = s.B.get(); a.k = k; a.lanes = s.lanesA;
+        spmv<double, double>(c, SpOp::RES, a);  // r = b - A x
+    }
+    fcg_norm(c, s.fb, s.r.get(), 0);           // delta0 = ||r||
+    prec->apply(s.r.get(), s.v.get(), k, true, nullptr);
+    {
+        SpmvArgs a;
+        a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        a.x = s.v.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA;
+        spmv<double, double>(c, SpOp::AX, a);  // w = A v
+    }
+    fcg_init_dots(c, s.fb, s.r.get(), s.v.get(), s.w.get());
+    fcg_init_update(c, s.fb, s.X.get(), s.r.get(), s.v.get(), s.w.get(), s.d.get(), s.q.get());
+    // ---- iterations, two at a time (the buffer roles alternate)
+    int check = std::max(2, p.check_every + (p.check_every & 1));
+    bool use_graph = p.use_graph != 0;
+    if (use_graph && !s.exec) {
+        BCMGX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            fcg_iteration(c, A, prec, s, 0);
+            fcg_iteration(c, A, prec, s, 1);
+        } catch (...) {
+            hipGraph_t g;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        BCMGX_HIP(hipStreamEndCapture(c->stream, &s.graph));
+        BCMGX_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+    }
+    int launched = 0;
+    int* hflag = (int*)c->host_scratch(sizeof(int) * 2);
+    while (true) {
+        for (int j = 0; j < check; j += 2) {
+            if (use_graph) BCMGX_HIP(hipGraphLaunch(s.exec, c->stream));
+            else {
+                fcg_iteration(c, A, prec, s, 0);
+                fcg_iteration(c, A, prec, s, 1);
+            }
+            launched += 2;
+        }
+        BCMGX_HIP(hipMemcpyAsync(hflag, s.flags.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        if (hflag[0]) break;
+        if (launched > p.itnlim + 2) break;  // the device stops every column at itnlim
+    }
+    copy_d2d(c, x, s.X.get(), sizeof(double) * nk);
+    std::vector<FcgState> st(k);
+    c->d2h(st.data(), s.st.get(), k);
+    SolveResult res;
+    prec->last_niter.assign(k, 0);
+    prec->last_exit.assign(k, 0);
+    prec->last_rel.assign(k, 0);
+    for (int j = 0; j < k; j++) {
+        int niter = st[j].iter + 1;  // FCG.cu:164
+        double rel = st[j].delta0 > 0 ? st[j].l2 / st[j].delta0 : 0.0;
+        prec->last_niter[j] = niter;
+        prec->last_exit[j] = st[j].l2;
+        prec->last_rel[j] = rel;
+        res.niter = std::max(res.niter, niter);
+        res.exit_res = std::max(res.exit_res, st[j].l2);
+        res.rel_res = std::max(res.rel_res, rel);
+        if (st[j].iter >= p.itnlim) res.retv = -1;  // FCG.cu:160-162
+        if (!std::isfinite(st[j].l2)) res.retv = BCMGX_ERR_NUMERIC;
+    }
+    res.del0 = st[0].delta0;
+    res.seconds = now_s() - t0;
+    return res;
+}
+
+}  // namespace bcmgx

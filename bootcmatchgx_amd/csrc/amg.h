@@ -1,0 +1,82 @@
+// amg.h -- hierarchy, preconditioners and the FCG driver (host side).
+#pragma once
+
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "kernels.h"
+
+namespace bcmgx {
+
+struct Level {
+    DCsr A;                 // owned (levels >= 1); level 0 borrows the user's matrix
+    const DCsr* Aref = nullptr;
+    DBuf<double> M;         // l1 row sums sum_j |a_ij| (relaxPrepare)
+    DBuf<float> Mf;
+    DBuf<int> agg;          // prolongator to the next level: one entry per row
+    DBuf<double> pval;
+    DBuf<float> pvalf;
+    DCsr R;                 // restrictor P^T (rows sorted by fine index)
+    int lanesA = 0, lanesR = 0;
+    const DCsr& mat() const { return Aref ? *Aref : A; }
+};
+
+struct Hierarchy {
+    std::vector<std::unique_ptr<Level>> L;
+    double op_cmplx = 1, op_wcmplx = 1, avg_cratio = 1;
+    int maxcs = 0;
+    int levels() const { return (int)L.size(); }
+};
+
+// cycle work vectors of one level, k columns interleaved
+template <typename V>
+struct LevelWork {
+    DBuf<V> F, X, X2, Res;
+};
+
+struct SolveWork;
+
+struct Prec {
+    Context* ctx = nullptr;
+    const DCsr* A = nullptr;
+    bcmgx_params p{};
+    int ptype = BCMGX_PREC_NONE;
+    Hierarchy H;
+    // one-level l1-Jacobi
+    DBuf<double> pl1j;
+    // AFSAI
+    DCsr G, GT;
+    int lanesG = 0, lanesGT = 0;
+    // work for the preconditioner application, sized for `kcap` columns
+    int kcap = 0;
+    std::vector<LevelWork<double>> wd;
+    std::vector<LevelWork<float>> wf;
+    DBuf<double> cd_F, cd_X, cd_X2;  // FP64 coarsest-level buffers of the FP32 cycle
+    DBuf<double> tmp;                // AFSAI intermediate / prec_apply staging
+    double setup_seconds = 0;
+    // solve workspaces per k (graphs are bound to their buffers)
+    std::vector<std::unique_ptr<SolveWork>> solves;
+    // per-column results of the last solve
+    std::vector<int> last_niter;
+    std::vector<double> last_exit, last_rel;
+
+    ~Prec();
+    void ensure_work(int k);
+    // u <- B r for k interleaved columns; `x_zero`: u holds zeros on entry (the FCG case)
+    void apply(const double* r, double* u, int k, bool x_zero, const int* stop);
+};
+
+std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& p);
+
+struct SolveResult {
+    int retv = 0, niter = 0;
+    double exit_res = 0, rel_res = 0, del0 = 0, seconds = 0;
+};
+// FCG on k interleaved columns (device pointers b, x; x in/out)
+SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
+
+// autotune: fastest lanes per row for SpMV with A (k = 1)
+int tune_lanes(Context* c, const DCsr& A, bool fp32);
+
+}  // namespace bcmgx

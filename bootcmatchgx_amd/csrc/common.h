@@ -1,0 +1,198 @@
+// common.h -- shared host/device infrastructure of the MI355X BootCMatchGX path.
+//
+// Errors never exit(): every failing HIP call throws bcmgx::Error, which the
+// C ABI layer turns into a status code plus a message stored on the context
+// (the reference exits from CHECK_DEVICE, src/utility/cudamacro.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bcmgx.h"
+
+namespace bcmgx {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define BCMGX_HIP(call)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw ::bcmgx::Error(BCMGX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + \
+                                                    " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+#define BCMGX_CHECK(cond, code, msg)                 \
+    do {                                             \
+        if (!(cond)) throw ::bcmgx::Error(code, msg); \
+    } while (0)
+
+// Kernel launch check (launch errors surface on the next API call otherwise).
+#define BCMGX_LAUNCHED() BCMGX_HIP(hipGetLastError())
+
+constexpr int WG = 256;  // default workgroup size (4 wave64s)
+
+inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
+
+// ---------------------------------------------------------------------------
+// Caching device allocator: hipMalloc'ed blocks are recycled by exact size
+// class (powers of two above 1 MiB granularity), so the setup's many
+// temporaries and repeated solves do not pay hipMalloc/hipFree.
+// ---------------------------------------------------------------------------
+class DevicePool {
+public:
+    explicit DevicePool(int device) : device_(device) {}
+    ~DevicePool() { release_all(); }
+    void* alloc(size_t bytes);
+    void free(void* p);
+    void release_all();
+    size_t bytes_in_use() const { return in_use_; }
+
+private:
+    static size_t size_class(size_t b);
+    int device_;
+    std::multimap<size_t, void*> free_;
+    std::map<void*, size_t> live_;
+    size_t in_use_ = 0;
+};
+
+struct Context;
+
+// Typed device buffer owned by a pool (RAII).
+template <typename T>
+class DBuf {
+public:
+    DBuf() = default;
+    DBuf(Context* c, size_t n) { reset(c, n); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept { swap(o); }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); swap(o); }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void reset(Context* c, size_t n);
+    void release();
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    explicit operator bool() const { return p_ != nullptr; }
+    void swap(DBuf& o) noexcept {
+        std::swap(ctx_, o.ctx_);
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
+
+private:
+    Context* ctx_ = nullptr;
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<DevicePool> pool;
+    int num_cus = 256;
+    int num_xcds = 8;
+    std::string last_error;
+    // pinned host staging for small device->host scalar reads
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    // autotune cache: matrix signature -> lanes per row
+    std::map<std::string, int> tuned_lanes;
+
+    explicit Context(int dev);
+    ~Context();
+    void sync() { BCMGX_HIP(hipStreamSynchronize(stream)); }
+    template <typename T>
+    T* dalloc(size_t n) { return static_cast<T*>(pool->alloc(n * sizeof(T))); }
+    void dfree(void* p) { pool->free(p); }
+    void* host_scratch(size_t bytes);
+    // copy n elements device->host through pinned memory and wait
+    template <typename T>
+    void d2h(T* dst, const T* src, size_t n);
+    template <typename T>
+    T read1(const T* src) {
+        T v;
+        d2h(&v, src, 1);
+        return v;
+    }
+    template <typename T>
+    void h2d(T* dst, const T* src, size_t n) {
+        BCMGX_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        sync();
+    }
+};
+
+template <typename T>
+void DBuf<T>::reset(Context* c, size_t n) {
+    release();
+    ctx_ = c;
+    n_ = n;
+    p_ = n ? c->dalloc<T>(n) : nullptr;
+}
+template <typename T>
+void DBuf<T>::release() {
+    if (p_ && ctx_) ctx_->dfree(p_);
+    p_ = nullptr;
+    n_ = 0;
+}
+
+template <typename T>
+void Context::d2h(T* dst, const T* src, size_t n) {
+    size_t b = n * sizeof(T);
+    void* h = host_scratch(b);
+    BCMGX_HIP(hipMemcpyAsync(h, src, b, hipMemcpyDeviceToHost, stream));
+    sync();
+    std::memcpy(dst, h, b);
+}
+
+// ---------------------------------------------------------------------------
+// Device CSR: int32 row pointers/columns, FP64 values, optional FP32 copy.
+// ---------------------------------------------------------------------------
+struct DCsr {
+    int n = 0, m = 0, nnz = 0;
+    DBuf<int> row, col;
+    DBuf<double> val;
+    DBuf<float> valf;  // FP32 copy for the mixed-precision cycle (may be empty)
+    void alloc(Context* c, int n_, int m_, int nnz_) {
+        n = n_; m = m_; nnz = nnz_;
+        row.reset(c, (size_t)n + 1);
+        col.reset(c, (size_t)(nnz > 0 ? nnz : 1));
+        val.reset(c, (size_t)(nnz > 0 ? nnz : 1));
+    }
+};
+
+}  // namespace bcmgx
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+#ifdef __HIPCC__
+namespace bcmgx {
+// Workgroups are dispatched round-robin over the 8 XCDs (WG b -> XCD b % 8),
+// each with its own L2.  Remap so XCD x owns one contiguous range of logical
+// blocks: rows that share x-vector gathers (neighbouring planes of a stencil)
+// then hit the same L2.  Bijective on [0, G).
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    const int X = 8;
+    int xcd = b % X, local = b / X;
+    int per = G / X, rem = G % X;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+template <typename T>
+__device__ __forceinline__ T ldg_nt(const T* p) { return __builtin_nontemporal_load(p); }
+}  // namespace bcmgx
+#endif

@@ -1,0 +1,269 @@
+// fcg.hip -- flexible CG vector kernels (flexibileConjugateGradients_v3,
+// src/solver/fcg/FCG.cu:27-175) with the scalar recurrences kept on the
+// device.  The reference reads every dot product back to the host
+// (triple_innerproduct, src/op/triple_inner_product.cu:64-101; norm_MPI,
+// src/datastruct/vector.cu:492-516); here the scalars, the convergence test
+// and the iteration count live in FcgState so a whole iteration can be
+// replayed from a hipGraph and the host only polls a flag.
+//
+// Reductions are deterministic: a fixed number of partial blocks (a function
+// of n only), a fixed in-block tree, then one block per column summing the
+// partials in a fixed tree.  k vectors are interleaved (x[i*k + c]); every
+// column has its own scalars and stop flag.
+#include "kernels.h"
+
+namespace bcmgx {
+
+int fcg_partial_blocks(int n) {
+    long long b = ((long long)n + 2047) / 2048;
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return (int)b;
+}
+
+static inline int kpad(int k) {
+    int p = 1;
+    while (p < k) p <<= 1;
+    return p;
+}
+
+// NV dot products per column over a row range; partials[(blk*k + c)*3 + m]
+template <int NV>
+__global__ __launch_bounds__(WG) void k_dots(int n, int k, int KP, const double* __restrict__ a0, const double* __restrict__ b0,
+                                             const double* __restrict__ a1, const double* __restrict__ b1,
+                                             const double* __restrict__ a2, const double* __restrict__ b2,
+                                             double* __restrict__ partials, const int* __restrict__ all_done) {
+    if (all_done && *all_done) return;
+    __shared__ double s[NV][WG];
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    double acc[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) acc[m] = 0.0;
+    if (c < k) {
+        for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+            long long o = i * k + c;
+            acc[0] += a0[o] * b0[o];
+            if (NV > 1) acc[1] += a1[o] * b1[o];
+            if (NV > 2) acc[2] += a2[o] * b2[o];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) s[m][tid] = acc[m];
+    __syncthreads();
+    for (int st = WG / 2; st >= KP; st >>= 1) {
+        if (tid < st) {
+#pragma unroll
+            for (int m = 0; m < NV; m++) s[m][tid] += s[m][tid + st];
+        }
+        __syncthreads();
+    }
+    if (tid < k) {
+#pragma unroll
+        for (int m = 0; m < NV; m++) partials[((long long)blockIdx.x * k + tid) * 3 + m] = s[m][tid];
+    }
+}
+
+// sum partials of column blockIdx.x in a fixed tree; thread 0 gets the totals
+template <int NV>
+__device__ void sum_partials(const double* __restrict__ partials, int nblk, int k, int c, double* out, double (*s)[WG]) {
+    int tid = threadIdx.x;
+    double acc[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) acc[m] = 0.0;
+    for (int b = tid; b < nblk; b += WG)
+#pragma unroll
+        for (int m = 0; m < NV; m++) acc[m] += partials[((long long)b * k + c) * 3 + m];
+#pragma unroll
+    for (int m = 0; m < NV; m++) s[m][tid] = acc[m];
+    __syncthreads();
+    for (int st = WG / 2; st > 0; st >>= 1) {
+        if (tid < st)
+#pragma unroll
+            for (int m = 0; m < NV; m++) s[m][tid] += s[m][tid + st];
+        __syncthreads();
+    }
+    if (tid == 0)
+#pragma unroll
+        for (int m = 0; m < NV; m++) out[m] = s[m][0];
+}
+
+// mode 0: initial residual norm (delta0); mode 1: post-initial-step norm
+__global__ __launch_bounds__(WG) void k_norm_final(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                                   int mode, int* all_done, int* ndone) {
+    __shared__ double s[1][WG];
+    double tot[1];
+    int c = blockIdx.x;
+    sum_partials<1>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        double l2 = sqrt(tot[0]);
+        st[c].l2 = l2;
+        if (mode == 0) {
+            st[c].delta0 = l2;
+            st[c].iter = 0;
+            st[c].done = 0;
+            if (l2 == 0.0) {  // b == A x already: nothing to iterate on
+                st[c].done = 1;
+                if (atomicAdd(ndone, 1) + 1 == k) *all_done = 1;
+            }
+        }
+    }
+}
+
+void fcg_norm(Context* c, const FcgBufs& f, const double* v, int mode) {
+    int KP = kpad(f.k);
+    k_dots<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, v, v, nullptr, nullptr, nullptr, nullptr, f.partials, nullptr);
+    BCMGX_LAUNCHED();
+    k_norm_final<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, mode, f.all_done, f.all_done + 1);
+    BCMGX_LAUNCHED();
+}
+
+__global__ __launch_bounds__(WG) void k_init_scalars(const double* __restrict__ partials, int nblk, int k, FcgState* st) {
+    __shared__ double s[2][WG];
+    double tot[2];
+    int c = blockIdx.x;
+    sum_partials<2>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        S.alpha = tot[0];
+        S.beta = tot[1];
+        S.delta = S.beta;                // FCG.cu:84
+        S.theta = S.done ? 0.0 : S.alpha / S.delta;  // FCG.cu:85
+        S.gamma = 0.0;
+    }
+}
+
+void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* v, const double* w) {
+    int KP = kpad(f.k);
+    k_dots<2><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, v, w, v, nullptr, nullptr, f.partials, nullptr);
+    BCMGX_LAUNCHED();
+    k_init_scalars<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st);
+    BCMGX_LAUNCHED();
+}
+
+__global__ __launch_bounds__(WG) void k_init_update(int n, int k, const FcgState* __restrict__ st, double* __restrict__ x,
+                                                    double* __restrict__ r, const double* __restrict__ v,
+                                                    const double* __restrict__ w, double* __restrict__ d,
+                                                    double* __restrict__ q) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * k) return;
+    int c = (int)(t % k);
+    double th = st[c].theta;
+    double vv = v[t], ww = w[t];
+    if (!st[c].done) {
+        x[t] += th * vv;      // my_axpby(v, x, theta, 1): y += a x  (basic.cu:114-130)
+        r[t] += -th * ww;     // my_axpby(w, r, -theta, 1)
+    }
+    d[t] = vv;                // d = clone(v), q = clone(w)  (FCG.cu:98-99)
+    q[t] = ww;
+}
+
+void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q) {
+    long long nk = (long long)f.n * f.k;
+    if (!nk) return;
+    k_init_update<<<grid_for(nk, WG), WG, 0, c->stream>>>(f.n, f.k, f.st, x, r, v, w, d, q);
+    BCMGX_LAUNCHED();
+}
+
+__global__ __launch_bounds__(WG) void k_scalars3(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                                 const int* __restrict__ all_done) {
+    if (*all_done) return;
+    __shared__ double s[3][WG];
+    double tot[3];
+    int c = blockIdx.x;
+    sum_partials<3>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        if (S.done) return;
+        S.alpha = tot[0];
+        S.beta = tot[1];
+        S.gamma = tot[2];
+        S.theta = S.gamma / S.delta;                        // FCG.cu:132
+        S.delta = S.beta - ((S.gamma * S.gamma) / S.delta);  // FCG.cu:135
+        S.theta2 = S.alpha / S.delta;                       // FCG.cu:136
+    }
+}
+
+void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1) {
+    int KP = kpad(f.k);
+    k_dots<3><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, p0, w0, p0, w1, p0, f.partials, f.all_done);
+    BCMGX_LAUNCHED();
+    k_scalars3<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.all_done);
+    BCMGX_LAUNCHED();
+}
+
+// double_merged_axpy(P1, P0, x, -theta, theta2); double_merged_axpy(W1, W0, r, -theta, -theta2)
+// (double_merged_axpy.cu:28-39), fused, with the partial sums of r.r.
+__global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgState* __restrict__ st,
+                                               double* __restrict__ p0, const double* __restrict__ p1,
+                                               double* __restrict__ w0, const double* __restrict__ w1,
+                                               double* __restrict__ x, double* __restrict__ r,
+                                               double* __restrict__ partials, const int* __restrict__ all_done) {
+    if (*all_done) return;
+    __shared__ double s[WG];
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    double acc = 0.0;
+    if (c < k) {
+        const FcgState S = st[c];
+        double mth = -S.theta, th2 = S.theta2, mth2 = -S.theta2;
+        for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+            long long o = i * k + c;
+            double rr;
+            if (!S.done) {
+                double t = mth * p1[o] + p0[o];
+                x[o] = th2 * t + x[o];
+                p0[o] = t;
+                double u = mth * w1[o] + w0[o];
+                rr = mth2 * u + r[o];
+                r[o] = rr;
+                w0[o] = u;
+            } else {
+                rr = r[o];
+            }
+            acc += rr * rr;
+        }
+    }
+    s[tid] = acc;
+    __syncthreads();
+    for (int stp = WG / 2; stp >= KP; stp >>= 1) {
+        if (tid < stp) s[tid] += s[tid + stp];
+        __syncthreads();
+    }
+    if (tid < k) partials[((long long)blockIdx.x * k + tid) * 3] = s[tid];
+}
+
+void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r) {
+    int KP = kpad(f.k);
+    k_update<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p0, p1, w0, w1, x, r, f.partials, f.all_done);
+    BCMGX_LAUNCHED();
+}
+
+// l2 = ||r||; iter++; stop when !(l2 > rtol*delta0 && iter < itnlim)  (FCG.cu:146-156)
+__global__ __launch_bounds__(WG) void k_finish(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                               double rtol, int itnlim, int* all_done, int* ndone) {
+    if (*all_done) return;
+    __shared__ double s[1][WG];
+    double tot[1];
+    int c = blockIdx.x;
+    sum_partials<1>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        if (S.done) return;
+        S.l2 = sqrt(tot[0]);
+        S.iter += 1;
+        if (!(S.l2 > rtol * S.delta0 && S.iter < itnlim)) {
+            S.done = 1;
+            if (atomicAdd(ndone, 1) + 1 == k) *all_done = 1;
+        }
+    }
+}
+
+void fcg_finish_iteration(Context* c, const FcgBufs& f) {
+    k_finish<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.rtol, f.itnlim, f.all_done, f.all_done + 1);
+    BCMGX_LAUNCHED();
+}
+
+}  // namespace bcmgx

@@ -1,0 +1,122 @@
+// kernels.h -- host-side launchers of the HIP kernels.  All launches go to
+// the context stream; none synchronises unless it returns a host value.
+#pragma once
+
+#include "common.h"
+
+namespace bcmgx {
+
+// ---- primitives (prims.hip) -------------------------------------------------
+// out[0..n) = exclusive prefix sums of in, out[n] = total; returns the total.
+int scan_exclusive(Context* c, const int* in, int* out, long long n);
+long long scan_exclusive(Context* c, const long long* in, long long* out, long long n);
+long long scan_exclusive_i32_to_i64(Context* c, const int* in, long long* out, long long n);
+template <typename T>
+void fill(Context* c, T* p, long long n, T v);
+template <typename A, typename B>
+void convert(Context* c, const A* s, B* d, long long n);
+void copy_d2d(Context* c, void* dst, const void* src, size_t bytes);
+void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k);
+void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k);
+double stream_copy_gbs(Context* c, size_t bytes, int reps);
+
+// ---- sparse matrix-vector family (spmv.hip) ---------------------------------
+// Vectors with k columns are interleaved row-major: x[i*k + c].
+enum class SpOp {
+    AX,      // y = A x                                   (CSR.cu:452-492, OP 1)
+    RES,     // y = b - A x                               (OP 2 applied to a copy of b)
+    AXPBY,   // y = alpha A x + beta y                    (OP 0)
+    JACOBI   // y = ((-(A x) + b) / D) + x                (_jacobi_it_full<0>, l1jacobi.cu:344-379)
+};
+
+struct SpmvArgs {
+    int n = 0;
+    const int* row = nullptr;
+    const int* col = nullptr;
+    const void* val = nullptr;  // T*
+    long long nnz = 0;
+    const void* x = nullptr;    // V*
+    void* y = nullptr;          // V*
+    const void* b = nullptr;    // V*  (RES, JACOBI)
+    const void* D = nullptr;    // V*  (JACOBI)
+    double alpha = 1.0, beta = 0.0;
+    int k = 1;                  // number of vector columns
+    int lanes = 0;              // lanes per row for k == 1 (0 = heuristic)
+    const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
+};
+// T = matrix value type, V = vector type: (double,double), (float,float), (float,double)
+template <typename T, typename V>
+void spmv(Context* c, SpOp op, const SpmvArgs& a);
+
+// lanes-per-row heuristic from nnz/row (wave64 adaptation of CSR.cu:614-652)
+int default_lanes(long long nnz, int n);
+
+// x_i = f_i / D_i  (first l1-Jacobi sweep from a zero guess)
+template <typename V>
+void jacobi_from_zero(Context* c, int n, int k, const V* f, const V* D, V* x, const int* stop);
+// mixed precision: F = (float) r; X = F / D   (level-0 entry of the FP32 cycle)
+void jacobi_from_zero_f64in(Context* c, int n, int k, const double* r, const float* D, float* F, float* X, const int* stop);
+// x_i = p_i * xc[agg_i] + x_i   (prolongation, OP 0 with alpha = beta = 1)
+template <typename V>
+void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const V* xc, V* x, const int* stop);
+template <typename A, typename B>
+void convert_stop(Context* c, const A* s, B* d, long long n, const int* stop);
+
+// ---- FCG kernels (fcg.hip) ----------------------------------------------------
+struct FcgState {
+    double delta0, delta, l2, alpha, beta, gamma, theta, theta2;
+    int iter, done, pad0, pad1;
+};
+struct FcgBufs {
+    int n = 0, k = 1;
+    int nblk = 0;              // partial-sum blocks (fixed for a given n)
+    double* partials = nullptr;  // nblk * k * 3
+    FcgState* st = nullptr;      // k states
+    int* all_done = nullptr;     // 1 when every column stopped
+    double rtol = 1e-6;
+    int itnlim = 2000;
+};
+int fcg_partial_blocks(int n);
+// ||v||^2 partials of each column -> st[c].l2 = sqrt(sum); mode 0: delta0 = l2 (initial)
+void fcg_norm(Context* c, const FcgBufs& f, const double* v, int mode);
+// alpha = r.v, beta = w.v -> delta = beta, theta = alpha / delta  (FCG.cu:56-86)
+void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* v, const double* w);
+// x += theta v; r -= theta w  (FCG.cu:88-89); d = v, q = w
+void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q);
+// (alpha, beta, gamma) = (r.p0, w0.p0, w1.p0), then theta, delta, theta2 (FCG.cu:113-136)
+void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1);
+// double_merged_axpy x2 (FCG.cu:138-144) fused with the ||r||^2 partials
+void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r);
+// ||r|| -> convergence test and iteration count (FCG.cu:146-156)
+void fcg_finish_iteration(Context* c, const FcgBufs& f);
+
+// ---- setup kernels (setup.hip) -------------------------------------------------
+// position of the first diagonal entry of each row (-1 if none)
+void diag_index(Context* c, int n, const int* row, const int* col, int* dind);
+// M_i = sum_j |a_ij| in column order (CSRm::absoluteRowSum, CSR.cu:1362-1397)
+void abs_row_sum(Context* c, int n, const int* row, const double* val, double* M);
+// l1 diagonal of the one-level preconditioner: a_ii + sum_{j!=i} |a_ij| (l1jacobi.cu:484-493)
+void l1_diag(Context* c, int n, const int* row, const int* col, const double* val, double* d);
+
+struct MatchWork;
+// suitor(): makeAH + toMaximumProductMatrix + _write_T_warp + Suitor (matching.cu:253-288)
+void suitor_match(Context* c, const DCsr& A, const double* w, int* M);
+// makeP_GPU with canonical singleton order; returns the coarse size
+int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval);
+// R = P^T for a one-entry-per-row P given as (agg, pval): rows sorted by fine index
+void transpose_prolongator(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& R);
+// general transpose (CSRm::transpose / Transpose_local)
+void transpose(Context* c, const DCsr& A, DCsr& T);
+// C = A B, deterministic ascending-k accumulation, sorted columns
+void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
+// P <- P P_i for one-entry-per-row operators: agg[k] = agg_i[agg[k]], pval[k] *= pval_i[agg[k]]
+void compose_prolongators(Context* c, int n, int* agg, double* pval, const int* agg_i, const double* pval_i);
+// csr of a one-entry-per-row prolongator (row = identity)
+void prolongator_csr(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& P);
+// g[k] = -sum_c lambda[row(k), c] x[col(k), c]
+void sparse_outer_grad(Context* c, const DCsr& A, const double* lambda, const double* x, int k, double* g);
+
+// ---- AFSAI (afsai.hip) ------------------------------------------------------------
+void afsai_build(Context* c, const DCsr& A, DCsr& G, DCsr& GT);
+
+}  // namespace bcmgx

@@ -1,0 +1,304 @@
+// prims.hip -- context, caching allocator and device primitives (scan,
+// fill, conversions).  The reference gets these from CUB/Thrust
+// (src/utility/devicePrefixSum.h, deviceMap.h, cuCompactor.cuh); here they are
+// written for wave64 directly.
+#include "kernels.h"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace bcmgx {
+
+// ------------------------------------------------------------------ pool ---
+size_t DevicePool::size_class(size_t b) {
+    if (b == 0) b = 1;
+    if (b <= (1u << 20)) {  // small: round to 4 KiB
+        return (b + 4095) & ~size_t(4095);
+    }
+    size_t c = size_t(1) << 20;
+    while (c < b) c <<= 1;
+    if (c - b > (c >> 2)) {  // large and far below the power of two: 1 MiB granularity
+        return (b + (1u << 20) - 1) & ~size_t((1u << 20) - 1);
+    }
+    return c;
+}
+
+void* DevicePool::alloc(size_t bytes) {
+    size_t c = size_class(bytes);
+    auto it = free_.find(c);
+    void* p = nullptr;
+    if (it != free_.end()) {
+        p = it->second;
+        free_.erase(it);
+    } else {
+        hipError_t e = hipMalloc(&p, c);
+        if (e != hipSuccess) {
+            // drop the cached blocks and retry once
+            (void)hipGetLastError();
+            for (auto& f : free_) (void)hipFree(f.second);
+            free_.clear();
+            e = hipMalloc(&p, c);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(BCMGX_ERR_ALLOC, "hipMalloc of " + std::to_string(c) + " bytes failed");
+            }
+        }
+    }
+    live_[p] = c;
+    in_use_ += c;
+    return p;
+}
+
+void DevicePool::free(void* p) {
+    if (!p) return;
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    in_use_ -= it->second;
+    free_.insert({it->second, p});
+    live_.erase(it);
+}
+
+void DevicePool::release_all() {
+    for (auto& f : free_) (void)hipFree(f.second);
+    free_.clear();
+    for (auto& l : live_) (void)hipFree(l.first);
+    live_.clear();
+    in_use_ = 0;
+}
+
+// --------------------------------------------------------------- context ---
+Context::Context(int dev) : device(dev) {
+    BCMGX_HIP(hipSetDevice(dev));
+    BCMGX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    pool.reset(new DevicePool(dev));
+    hipDeviceProp_t prop;
+    BCMGX_HIP(hipGetDeviceProperties(&prop, dev));
+    num_cus = prop.multiProcessorCount;
+}
+
+Context::~Context() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    pool.reset();
+    if (pinned) (void)hipHostFree(pinned);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void* Context::host_scratch(size_t bytes) {
+    if (bytes > pinned_bytes) {
+        if (pinned) BCMGX_HIP(hipHostFree(pinned));
+        pinned = nullptr;
+        size_t b = std::max<size_t>(bytes, 1 << 16);
+        BCMGX_HIP(hipHostMalloc(&pinned, b, hipHostMallocDefault));
+        pinned_bytes = b;
+    }
+    return pinned;
+}
+
+// ------------------------------------------------------------------ scan ---
+// Three-phase exclusive scan: per-block totals, scan of the totals in one
+// workgroup, then per-block rescan with the block offset.  1024 items per
+// block (256 threads x 4).
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_BLOCK = WG * SCAN_ITEMS;
+
+template <typename T>
+__device__ T block_exclusive_scan(T v, T* lds_wave, T& total) {
+    // wave64 inclusive scan with shuffles, then across the 4 waves through LDS
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds_wave[wid] = x;
+    __syncthreads();
+    T off = 0;
+    for (int w = 0; w < wid; w++) off += lds_wave[w];
+    total = lds_wave[0] + lds_wave[1] + lds_wave[2] + lds_wave[3];
+    __syncthreads();
+    return off + x - v;
+}
+
+template <typename TI, typename T>
+__global__ __launch_bounds__(WG) void k_scan_partials(const TI* __restrict__ in, long long n, T* __restrict__ partials) {
+    __shared__ T lw[4];
+    long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) s += (T)in[base + k];
+    T tot;
+    block_exclusive_scan<T>(s, lw, tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_scan_single(T* __restrict__ data, long long n, T* __restrict__ total_out) {
+    __shared__ T lw[4];
+    T carry = 0;
+    for (long long base = 0; base < n; base += SCAN_BLOCK) {
+        long long b = base + threadIdx.x * SCAN_ITEMS;
+        T v[SCAN_ITEMS];
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            v[k] = (b + k < n) ? data[b + k] : (T)0;
+            s += v[k];
+        }
+        T tot;
+        T ex = block_exclusive_scan<T>(s, lw, tot);
+        T run = carry + ex;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            if (b + k < n) data[b + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <typename TI, typename T>
+__global__ __launch_bounds__(WG) void k_scan_final(const TI* __restrict__ in, long long n, const T* __restrict__ offs, T* __restrict__ out) {
+    __shared__ T lw[4];
+    long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < n) ? (T)in[base + k] : (T)0;
+        s += v[k];
+    }
+    T tot;
+    T run = offs[blockIdx.x] + block_exclusive_scan<T>(s, lw, tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+template <typename TI, typename T>
+static T scan_impl(Context* c, const TI* in, T* out, long long n) {
+    // out[0..n) exclusive prefix sums, out[n] = total; returns the total.
+    if (n <= 0) {
+        T z = 0;
+        BCMGX_HIP(hipMemcpyAsync(out, &z, sizeof(T), hipMemcpyHostToDevice, c->stream));
+        c->sync();
+        return 0;
+    }
+    long long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    DBuf<T> part(c, nb + 1);
+    k_scan_partials<TI, T><<<(unsigned)nb, WG, 0, c->stream>>>(in, n, part.get());
+    BCMGX_LAUNCHED();
+    k_scan_single<T><<<1, WG, 0, c->stream>>>(part.get(), nb, out + n);
+    BCMGX_LAUNCHED();
+    k_scan_final<TI, T><<<(unsigned)nb, WG, 0, c->stream>>>(in, n, part.get(), out);
+    BCMGX_LAUNCHED();
+    return c->read1(out + n);
+}
+
+int scan_exclusive(Context* c, const int* in, int* out, long long n) { return scan_impl<int, int>(c, in, out, n); }
+long long scan_exclusive(Context* c, const long long* in, long long* out, long long n) {
+    return scan_impl<long long, long long>(c, in, out, n);
+}
+long long scan_exclusive_i32_to_i64(Context* c, const int* in, long long* out, long long n) {
+    return scan_impl<int, long long>(c, in, out, n);
+}
+
+// ------------------------------------------------------------ elementwise ---
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, long long n, T v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+template <typename T>
+void fill(Context* c, T* p, long long n, T v) {
+    if (n <= 0) return;
+    unsigned g = std::min<long long>(grid_for(n, WG), 65536);
+    k_fill<T><<<g, WG, 0, c->stream>>>(p, n, v);
+    BCMGX_LAUNCHED();
+}
+template void fill<int>(Context*, int*, long long, int);
+template void fill<double>(Context*, double*, long long, double);
+template void fill<float>(Context*, float*, long long, float);
+template void fill<long long>(Context*, long long*, long long, long long);
+
+template <typename A, typename B>
+__global__ void k_convert(const A* __restrict__ s, B* __restrict__ d, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = (B)s[i];
+}
+template <typename A, typename B>
+void convert(Context* c, const A* s, B* d, long long n) {
+    if (n <= 0) return;
+    unsigned g = std::min<long long>(grid_for(n, WG), 65536);
+    k_convert<A, B><<<g, WG, 0, c->stream>>>(s, d, n);
+    BCMGX_LAUNCHED();
+}
+template void convert<double, float>(Context*, const double*, float*, long long);
+template void convert<float, double>(Context*, const float*, double*, long long);
+template void convert<double, double>(Context*, const double*, double*, long long);
+
+void copy_d2d(Context* c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    BCMGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// column-major (ld) <-> interleaved row-major [n][k]
+template <typename T>
+__global__ void k_cm_to_rm(const T* __restrict__ s, int ld, T* __restrict__ d, int n, int k) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * k) return;
+    int i = (int)(t / k), col = (int)(t % k);
+    d[t] = s[(long long)col * ld + i];
+}
+template <typename T>
+__global__ void k_rm_to_cm(const T* __restrict__ s, T* __restrict__ d, int ld, int n, int k) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * k) return;
+    int i = (int)(t / k), col = (int)(t % k);
+    d[(long long)col * ld + i] = s[t];
+}
+void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k) {
+    if ((long long)n * k == 0) return;
+    k_cm_to_rm<double><<<grid_for((long long)n * k, WG), WG, 0, c->stream>>>(s, ld, d, n, k);
+    BCMGX_LAUNCHED();
+}
+void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k) {
+    if ((long long)n * k == 0) return;
+    k_rm_to_cm<double><<<grid_for((long long)n * k, WG), WG, 0, c->stream>>>(s, d, ld, n, k);
+    BCMGX_LAUNCHED();
+}
+
+// sustained copy bandwidth (STREAM-copy style, 16 B per lane per access)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(WG) void k_copy16(const dvec2* __restrict__ s, dvec2* __restrict__ d, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(&s[i]), &d[i]);
+}
+double stream_copy_gbs(Context* c, size_t bytes, int reps) {
+    long long n = (long long)(bytes / 16);
+    DBuf<dvec2> a(c, n), b(c, n);
+    BCMGX_HIP(hipMemsetAsync(a.get(), 0, n * 16, c->stream));
+    unsigned g = std::min<long long>(grid_for(n, WG), (long long)c->num_cus * 32);
+    hipEvent_t e0, e1;
+    BCMGX_HIP(hipEventCreate(&e0));
+    BCMGX_HIP(hipEventCreate(&e1));
+    k_copy16<<<g, WG, 0, c->stream>>>(a.get(), b.get(), n);
+    BCMGX_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) k_copy16<<<g, WG, 0, c->stream>>>(a.get(), b.get(), n);
+    BCMGX_HIP(hipEventRecord(e1, c->stream));
+    BCMGX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    BCMGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 2.0 * n * 16.0 * reps / (ms * 1e-3) / 1e9;
+}
+
+}  // namespace bcmgx

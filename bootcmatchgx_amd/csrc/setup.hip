@@ -1,0 +1,828 @@
+// setup.hip -- AMG setup kernels: matching weights, Suitor matching,
+// prolongator, transposes and the Galerkin SpGEMM.
+//
+// Everything here is bit-reproducible and agrees bit-for-bit with the CPU
+// oracle: floating-point expressions are written in the reference's operation
+// order and compiled without FMA contraction, and no floating-point value is
+// ever accumulated through atomics.
+#include "kernels.h"
+
+#include <algorithm>
+#include <cfloat>
+
+namespace bcmgx {
+
+// ------------------------------------------------------------ diagonals ---
+__global__ void k_diag_index(int n, const int* __restrict__ row, const int* __restrict__ col, int* __restrict__ dind) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = -1;
+    for (int j = row[i]; j < row[i + 1]; j++)
+        if (col[j] == i) { d = j; break; }
+    dind[i] = d;
+}
+void diag_index(Context* c, int n, const int* row, const int* col, int* dind) {
+    if (n <= 0) return;
+    k_diag_index<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, col, dind);
+    BCMGX_LAUNCHED();
+}
+
+// CSRm::absoluteRowSum (CSR.cu:1362-1397), column order
+__global__ void k_abs_row_sum(int n, const int* __restrict__ row, const double* __restrict__ val, double* __restrict__ M) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    for (int j = row[i]; j < row[i + 1]; j++) t += fabs(val[j]);
+    M[i] = t;
+}
+void abs_row_sum(Context* c, int n, const int* row, const double* val, double* M) {
+    if (n <= 0) return;
+    k_abs_row_sum<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, val, M);
+    BCMGX_LAUNCHED();
+}
+
+// set_l1j (l1jacobi.cu:484-493): mydiag then add_absrowsum_nodiag
+__global__ void k_l1_diag(int n, const int* __restrict__ row, const int* __restrict__ col, const double* __restrict__ val,
+                          double* __restrict__ d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dg = 0.0, s = 0.0;
+    for (int j = row[i]; j < row[i + 1]; j++)
+        if (col[j] == i) dg = val[j];
+    for (int j = row[i]; j < row[i + 1]; j++)
+        if (col[j] != i) s += fabs(val[j]);
+    d[i] = dg + s;
+}
+void l1_diag(Context* c, int n, const int* row, const int* col, const double* val, double* d) {
+    if (n <= 0) return;
+    k_l1_diag<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, col, val, d);
+    BCMGX_LAUNCHED();
+}
+
+// ------------------------------------------------------ matching weights ---
+constexpr double SUITOR_EPS = 3.885780586188048e-10;  // suitor.h:8
+
+// _makeC (matching.cu:113-135): C_i = a_ii w_i w_i
+__global__ void k_make_C(int n, const double* __restrict__ val, const int* __restrict__ dind, const double* __restrict__ w,
+                         double* __restrict__ C) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    C[i] = val[dind[i]] * w[i] * w[i];
+}
+
+// _makeAH_warp (matching.cu:66-111): off-diagonal AH_ij = 1 - 2 a_ij w_i w_j / (C_i + C_j),
+// stored on A's pattern without the diagonal (row i of W starts at row[i] - i).
+template <int L>
+__global__ __launch_bounds__(WG) void k_make_AH(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                const double* __restrict__ val, const int* __restrict__ dind,
+                                                const double* __restrict__ w, const double* __restrict__ C,
+                                                int* __restrict__ Wcol, double* __restrict__ Wval) {
+    int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
+    int lane = (int)threadIdx.x % L;
+    if (i >= n) return;
+    int s = row[i], e = row[i + 1], d = dind[i];
+    for (int j = s + lane; j < e; j += L) {
+        if (j == d) continue;
+        int cc = col[j];
+        int o = j - i - (j > d ? 1 : 0);
+        int lo = cc < i ? cc : i, hi = cc < i ? i : cc;
+        double norm = C[lo] + C[hi];
+        double v;
+        if (norm > DBL_EPSILON) {
+            double wt = w[lo] * w[hi];
+            v = 1. - ((2. * val[j] * wt) / norm);
+        } else {
+            v = DBL_EPSILON;
+        }
+        Wcol[o] = cc;
+        Wval[o] = v;
+    }
+}
+
+// find_Max_Min(op 1) (matching.cu:188-217): min |AH| (a min is order independent)
+__global__ __launch_bounds__(WG) void k_absmin(long long m, const double* __restrict__ v, unsigned long long* __restrict__ out) {
+    __shared__ double s[WG];
+    double mn = DBL_MAX;
+    for (long long t = (long long)blockIdx.x * WG + threadIdx.x; t < m; t += (long long)gridDim.x * WG) mn = fmin(mn, fabs(v[t]));
+    s[threadIdx.x] = mn;
+    __syncthreads();
+    for (int st = WG / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + st]);
+        __syncthreads();
+    }
+    // non-negative doubles order like their bit patterns
+    if (threadIdx.x == 0) atomicMin(out, (unsigned long long)__double_as_longlong(s[0]));
+}
+
+// _make_w (matching.cu:219-229) with toMaximumProductMatrix's clamp (:241-245)
+__global__ void k_make_W(long long m, double* __restrict__ v, const unsigned long long* __restrict__ minbits) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    double mn = __longlong_as_double((long long)*minbits);
+    if (fabs(mn) < DBL_EPSILON) mn = DBL_EPSILON;
+    double s = fabs(v[t]);
+    v[t] = log(s != 0.0 ? s : SUITOR_EPS / (0.999 * mn));
+}
+
+__device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// _write_T_warp (matching.cu:29-64): W(i, c) <- W(c, i) for c < i
+template <int L>
+__global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
+                                                   double* __restrict__ Wval) {
+    int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
+    int lane = (int)threadIdx.x % L;
+    if (i >= n) return;
+    for (int j = Wrow[i] + lane; j < Wrow[i + 1]; j += L) {
+        int cc = Wcol[j];
+        if (cc >= i) break;
+        int b = Wrow[cc];
+        int p = lower_bound_dev(Wcol + b, Wrow[cc + 1] - b, i);
+        Wval[j] = Wval[b + p];
+    }
+}
+
+// Preference ranks.  Vertex y orders its neighbours u by (W(y,u), u) -- the
+// Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
+// rank[e] = position of entry e of row y in that order; inv[Wrow[y] + r] = u.
+__global__ void k_rank(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol, const double* __restrict__ Wval,
+                       int* __restrict__ rank, int* __restrict__ inv) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= n) return;
+    int s = Wrow[y], e = Wrow[y + 1];
+    for (int a = s; a < e; a++) {
+        double wa = Wval[a];
+        int ca = Wcol[a];
+        int r = 0;
+        for (int b = s; b < e; b++) {
+            double wb = Wval[b];
+            r += (wb < wa || (wb == wa && Wcol[b] < ca)) ? 1 : 0;
+        }
+        rank[a] = r;
+        inv[s + r] = ca;
+    }
+}
+
+// tkey[e] for entry e = (u, y) of row u: 1 + rank of u among y's neighbours
+__global__ void k_tkey(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol, const int* __restrict__ rank,
+                       int* __restrict__ tkey) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    for (int e = Wrow[u]; e < Wrow[u + 1]; e++) {
+        int y = Wcol[e];
+        int b = Wrow[y];
+        int p = lower_bound_dev(Wcol + b, Wrow[y + 1] - b, u);
+        tkey[e] = rank[b + p] + 1;
+    }
+}
+
+// Lock-free Suitor.  S[y] holds 1 + the rank of y's current suitor (0: none).
+// A vertex proposes to its best neighbour -- positive weight, heaviest, ties
+// to the higher id (find_Candidate, suitor.cu:80-147) -- that would prefer it
+// to its current suitor; atomicMax installs the offer and returns the
+// displaced suitor, which this thread then re-proposes (set_suitor,
+// suitor.cu:51-78, without the spin locks).  Every failed atomicMax means
+// another offer raised S[best], and offers only rise, so the grid drains.
+// The fixed point is the unique stable (greedy) matching of the total order
+// (W, max id, min id), whatever the interleaving.
+__global__ void k_suitor(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol, const double* __restrict__ Wval,
+                         const int* __restrict__ tkey, const int* __restrict__ inv, int* S) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    int cur = u;
+    while (cur >= 0) {
+        int best = -1, bkey = 0;
+        double bw = 0.0;
+        for (int e = Wrow[cur]; e < Wrow[cur + 1]; e++) {
+            double wv = Wval[e];
+            if (!(wv > 0)) continue;
+            int y = Wcol[e];
+            if (y < 0 || y >= n) continue;
+            int k = tkey[e];
+            int sy = __hip_atomic_load(&S[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (sy >= k) continue;
+            if (best < 0 || wv > bw || (wv == bw && y > best)) {
+                best = y;
+                bw = wv;
+                bkey = k;
+            }
+        }
+        if (best < 0) break;
+        int old = atomicMax(&S[best], bkey);
+        if (old < bkey) cur = old > 0 ? inv[Wrow[best] + old - 1] : -1;
+    }
+}
+
+__global__ void k_suitor_decode(int n, const int* __restrict__ Wrow, const int* __restrict__ inv, const int* __restrict__ S,
+                                int* __restrict__ M) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= n) return;
+    int s = S[y];
+    M[y] = s > 0 ? inv[Wrow[y] + s - 1] : -1;
+}
+
+
+__global__ void k_make_wrow(int n, const int* __restrict__ row, int* __restrict__ Wrow) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) Wrow[i] = row[i] - i;
+}
+
+static int lanes_for(long long nnz, int n) {
+    double d = n > 0 ? (double)nnz / n : 1;
+    if (d <= 4) return 4;
+    if (d <= 8) return 8;
+    if (d <= 16) return 16;
+    return 32;
+}
+
+void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
+    int n = A.n;
+    if (n <= 0) return;
+    long long m = (long long)A.nnz - n;  // off-diagonal entries (one diagonal per row, checked by the caller)
+    size_t mm = (size_t)std::max<long long>(m, 1);
+    DBuf<int> dind(c, n);
+    diag_index(c, n, A.row.get(), A.col.get(), dind.get());
+    DBuf<double> C(c, n);
+    k_make_C<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.val.get(), dind.get(), w, C.get());
+    BCMGX_LAUNCHED();
+    DBuf<int> Wrow(c, (size_t)n + 1), Wcol(c, mm);
+    DBuf<double> Wval(c, mm);
+    k_make_wrow<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, A.row.get(), Wrow.get());
+    BCMGX_LAUNCHED();
+    int L = lanes_for(A.nnz, n);
+#define BCMGX_AH(LL)                                                                                               \
+    case LL:                                                                                                       \
+        k_make_AH<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(),      \
+                                                                  dind.get(), w, C.get(), Wcol.get(), Wval.get()); \
+        break;
+    switch (L) { BCMGX_AH(4) BCMGX_AH(8) BCMGX_AH(16) BCMGX_AH(32) }
+#undef BCMGX_AH
+    BCMGX_LAUNCHED();
+    DBuf<unsigned long long> mn(c, 1);
+    double big = DBL_MAX;
+    unsigned long long bigbits;
+    std::memcpy(&bigbits, &big, sizeof(big));
+    BCMGX_HIP(hipMemcpyAsync(mn.get(), &bigbits, sizeof(bigbits), hipMemcpyHostToDevice, c->stream));
+    if (m > 0) {
+        unsigned g = (unsigned)std::min<long long>(grid_for(m, WG), 2048);
+        k_absmin<<<g, WG, 0, c->stream>>>(m, Wval.get(), mn.get());
+        BCMGX_LAUNCHED();
+        k_make_W<<<grid_for(m, WG), WG, 0, c->stream>>>(m, Wval.get(), mn.get());
+        BCMGX_LAUNCHED();
+    }
+#define BCMGX_SY(LL)                                                                                         \
+    case LL:                                                                                                 \
+        k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get()); \
+        break;
+    switch (L) { BCMGX_SY(4) BCMGX_SY(8) BCMGX_SY(16) BCMGX_SY(32) }
+#undef BCMGX_SY
+    BCMGX_LAUNCHED();
+    DBuf<int> rank(c, mm), inv(c, mm), tkey(c, mm);
+    k_rank<<<grid_for(n, 64), 64, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(), inv.get());
+    BCMGX_LAUNCHED();
+    k_tkey<<<grid_for(n, 64), 64, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), tkey.get());
+    BCMGX_LAUNCHED();
+    DBuf<int> S(c, n);
+    fill<int>(c, S.get(), n, 0);
+    k_suitor<<<grid_for(n, 64), 64, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get());
+    BCMGX_LAUNCHED();
+    k_suitor_decode<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Wrow.get(), inv.get(), S.get(), M);
+    BCMGX_LAUNCHED();
+}
+
+// ------------------------------------------------------------ prolongator ---
+// makeP_GPU (matchingPairAggregation.cu:109-170).  Pair (v < u = M[v]) is good
+// when ||(w_v, w_u)|| > eps (_aggregate_symmetric_unsorted, :47-79); good
+// pairs take ids in master order (DeviceSelect::If, :127-148 +
+// _aggregate_symmetric_sort, :81-93).  Singletons are numbered in ascending
+// index (the reference uses atomicAdd order, :38): good -> next id with
+// P = w/|w|, bad -> (running count) - 1 with P = 0 (:41-44).
+__global__ void k_pair_flags(int n, const int* __restrict__ M, const double* __restrict__ w, int* __restrict__ good,
+                             double* __restrict__ pval) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int u = M[v];
+    int g = 0;
+    if (u >= 0 && v < u) {
+        double wv = w[v], wu = w[u];
+        double nrm = sqrt(wv * wv + wu * wu);
+        if (nrm > DBL_EPSILON) {
+            g = 1;
+            pval[v] = wv / nrm;
+            pval[u] = wu / nrm;
+        }
+    }
+    good[v] = g;
+}
+__global__ void k_assign_pairs(int n, const int* __restrict__ M, const int* __restrict__ good, const int* __restrict__ pid,
+                               int* __restrict__ agg) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    if (good[v]) {
+        agg[v] = pid[v];
+        agg[M[v]] = pid[v];
+    }
+}
+__global__ void k_single_flags(int n, const int* __restrict__ M, const int* __restrict__ good, const double* __restrict__ w,
+                               int* __restrict__ single, int* __restrict__ inpair) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int u = M[v];
+    int ip = good[v] || (u >= 0 && u < v && good[u]);
+    inpair[v] = ip;
+    single[v] = (!ip && fabs(w[v]) > DBL_EPSILON) ? 1 : 0;
+}
+__global__ void k_assign_singles(int n, int npairs, const int* __restrict__ inpair, const int* __restrict__ sid,
+                                 const double* __restrict__ w, int* __restrict__ agg, double* __restrict__ pval) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    if (inpair[v]) return;
+    double wv = w[v];
+    if (fabs(wv) > DBL_EPSILON) {
+        agg[v] = npairs + sid[v];
+        pval[v] = wv / fabs(wv);
+    } else {
+        int a = npairs + sid[v] - 1;
+        agg[v] = a > 0 ? a : 0;
+        pval[v] = 0.0;
+    }
+}
+
+int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval) {
+    if (n <= 0) return 0;
+    DBuf<int> good(c, n), pid(c, (size_t)n + 1), single(c, n), inpair(c, n), sid(c, (size_t)n + 1);
+    k_pair_flags<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, w, good.get(), pval);
+    BCMGX_LAUNCHED();
+    int npairs = scan_exclusive(c, good.get(), pid.get(), n);
+    k_assign_pairs<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, good.get(), pid.get(), agg);
+    BCMGX_LAUNCHED();
+    k_single_flags<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, good.get(), w, single.get(), inpair.get());
+    BCMGX_LAUNCHED();
+    int nsingles = scan_exclusive(c, single.get(), sid.get(), n);
+    k_assign_singles<<<grid_for(n, WG), WG, 0, c->stream>>>(n, npairs, inpair.get(), sid.get(), w, agg, pval);
+    BCMGX_LAUNCHED();
+    return npairs + nsingles;
+}
+
+// P <- P P_i for one-entry-per-row operators (nsparseMGPU(P, P_i), matchingAggregation.cu:175-191)
+__global__ void k_compose(int n, int* __restrict__ agg, double* __restrict__ pval, const int* __restrict__ agg_i,
+                          const double* __restrict__ pval_i) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int a = agg[k];
+    pval[k] = pval[k] * pval_i[a];
+    agg[k] = agg_i[a];
+}
+void compose_prolongators(Context* c, int n, int* agg, double* pval, const int* agg_i, const double* pval_i) {
+    if (n <= 0) return;
+    k_compose<<<grid_for(n, WG), WG, 0, c->stream>>>(n, agg, pval, agg_i, pval_i);
+    BCMGX_LAUNCHED();
+}
+
+__global__ void k_iota(int n, int* __restrict__ r) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) r[i] = i;
+}
+void prolongator_csr(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& P) {
+    P.alloc(c, n, nc, n);
+    k_iota<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, P.row.get());
+    BCMGX_LAUNCHED();
+    copy_d2d(c, P.col.get(), agg, sizeof(int) * n);
+    copy_d2d(c, P.val.get(), pval, sizeof(double) * n);
+}
+
+// ------------------------------------------------------------- transpose ---
+// CSRm::Transpose_local (CSR.cu:1752-1822): entries ordered by (col, row).
+// Count per column, scan, scatter with atomic cursors, then sort each output
+// row by source row so the result is deterministic.
+__global__ void k_count_cols(long long nnz, const int* __restrict__ col, int* __restrict__ cnt) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nnz) atomicAdd(&cnt[col[t]], 1);
+}
+__global__ void k_scatter_T(int n, const int* __restrict__ row, const int* __restrict__ col, const double* __restrict__ val,
+                            const int* __restrict__ trow, int* __restrict__ cursor, int* __restrict__ tcol,
+                            double* __restrict__ tval) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int j = row[i]; j < row[i + 1]; j++) {
+        int cc = col[j];
+        int p = trow[cc] + atomicAdd(&cursor[cc], 1);
+        tcol[p] = i;
+        tval[p] = val[j];
+    }
+}
+__global__ void k_sort_rows(int n, const int* __restrict__ row, int* __restrict__ col, double* __restrict__ val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = row[i], e = row[i + 1];
+    for (int a = s + 1; a < e; a++) {  // insertion sort (rows are short)
+        int kc = col[a];
+        double kv = val[a];
+        int b = a - 1;
+        while (b >= s && col[b] > kc) {
+            col[b + 1] = col[b];
+            val[b + 1] = val[b];
+            b--;
+        }
+        col[b + 1] = kc;
+        val[b + 1] = kv;
+    }
+}
+static void transpose_raw(Context* c, int n, int m, long long nnz, const int* row, const int* col, const double* val, DCsr& T) {
+    T.alloc(c, m, n, (int)nnz);
+    DBuf<int> cnt(c, (size_t)m + 1), cursor(c, (size_t)m + 1);
+    fill<int>(c, cnt.get(), m + 1, 0);
+    fill<int>(c, cursor.get(), m + 1, 0);
+    if (nnz > 0) {
+        k_count_cols<<<grid_for(nnz, WG), WG, 0, c->stream>>>(nnz, col, cnt.get());
+        BCMGX_LAUNCHED();
+    }
+    scan_exclusive(c, cnt.get(), T.row.get(), m);
+    if (n > 0) {
+        k_scatter_T<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, col, val, T.row.get(), cursor.get(), T.col.get(), T.val.get());
+        BCMGX_LAUNCHED();
+    }
+    if (m > 0) {
+        k_sort_rows<<<grid_for(m, 64), 64, 0, c->stream>>>(m, T.row.get(), T.col.get(), T.val.get());
+        BCMGX_LAUNCHED();
+    }
+}
+void transpose(Context* c, const DCsr& A, DCsr& T) { transpose_raw(c, A.n, A.m, A.nnz, A.row.get(), A.col.get(), A.val.get(), T); }
+void transpose_prolongator(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& R) {
+    DBuf<int> row(c, (size_t)n + 1);
+    k_iota<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, row.get());
+    BCMGX_LAUNCHED();
+    transpose_raw(c, n, nc, n, row.get(), agg, pval, R);
+}
+
+// ---------------------------------------------------------------- SpGEMM ---
+// C = A B row by row (replaces nsparse's two-phase hash SpGEMM,
+// kernel_spgemm_hash_d.cu:1196-1464).  Each row's products are generated in
+// order (A entry, then B entry), tagged key = (column << 32) | sequence, sorted
+// (bitonic) in LDS, and every run of equal columns is summed left to right:
+// C_ij = sum over k in ascending position, the order of the oracle's
+// Gustavson loop.  Rows are binned by product count:
+//   <= 64 products     : one wave64 per row, sort in registers via shuffles;
+//   <= 2048 products   : one workgroup per row, sort in LDS;
+//   larger             : one workgroup per row, sort in global scratch.
+constexpr int SMALL_P = 64;
+constexpr int MED_P = 2048;  // 2048 x (8 + 8 + 4) B + scan = 41 KiB of LDS
+
+__global__ void k_count_products(int n, const int* __restrict__ arow, const int* __restrict__ acol, const int* __restrict__ brow,
+                                 long long* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long s = 0;
+    for (int j = arow[i]; j < arow[i + 1]; j++) {
+        int k = acol[j];
+        s += brow[k + 1] - brow[k];
+    }
+    cnt[i] = s;
+}
+
+__global__ void k_bin_rows(int n, const int* __restrict__ arow, const long long* __restrict__ cnt, int* __restrict__ bins,
+                           int* __restrict__ bincount, int cap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long p = cnt[i];
+    int alen = arow[i + 1] - arow[i];
+    int b = (p <= SMALL_P && alen <= SMALL_P) ? 0 : ((p <= MED_P && alen <= MED_P) ? 1 : 2);
+    int pos = atomicAdd(&bincount[b], 1);
+    bins[(long long)b * cap + pos] = i;
+}
+
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    return (unsigned long long)__shfl_xor((long long)v, m, 64);
+}
+
+// one wave64 per row, products <= 64
+__global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
+                                                     const int* __restrict__ acol, const double* __restrict__ aval,
+                                                     const int* __restrict__ brow, const int* __restrict__ bcol,
+                                                     const double* __restrict__ bval, const long long* __restrict__ poff,
+                                                     int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq) {
+    __shared__ int pre_s[4][SMALL_P + 1];
+    __shared__ double v_s[4][SMALL_P];
+    int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int w = blockIdx.x * 4 + wid;
+    bool active = w < nrows;  // inactive waves still take part in the barriers
+    int i = active ? rows[w] : 0;
+    int s = active ? arow[i] : 0, alen = active ? arow[i + 1] - s : 0;
+    int k = -1, cntl = 0;
+    if (lane < alen) {
+        k = acol[s + lane];
+        cntl = brow[k + 1] - brow[k];
+    }
+    // inclusive scan of B-row lengths across the wave
+    int x = cntl;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    int total = __shfl(x, 63, 64);
+    pre_s[wid][lane + 1] = x;
+    if (lane == 0) pre_s[wid][0] = 0;
+    __syncthreads();
+    unsigned long long key = ~0ull;
+    double v = 0.0;
+    if (lane < total) {
+        // owner: largest a with pre[a] <= lane
+        int lo = 0, hi = alen;  // pre[0] = 0 <= lane
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (pre_s[wid][mid] <= lane) lo = mid;
+            else hi = mid;
+        }
+        int a = lo;
+        int ka = acol[s + a];
+        int jb = brow[ka] + (lane - pre_s[wid][a]);
+        int cc = bcol[jb];
+        v = aval[s + a] * bval[jb];
+        key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)lane;
+    }
+    // bitonic sort of 64 (key, v) pairs across the wave
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            unsigned long long pk = shfl_xor_u64(key, j);
+            double pv = __shfl_xor(v, j, 64);
+            bool up = (lane & kk) == 0;
+            bool lower = (lane & j) == 0;
+            bool take = (lower == up) ? (pk < key) : (pk > key);
+            if (take) {
+                key = pk;
+                v = pv;
+            }
+        }
+    }
+    v_s[wid][lane] = v;
+    int colv = (int)(key >> 32);
+    int prevc = __shfl_up(colv, 1, 64);
+    bool head = lane < total && (lane == 0 || prevc != colv);
+    unsigned long long hm = __ballot(head);
+    __syncthreads();
+    if (head && active) {
+        double acc = v;
+        // sum the run left to right
+        for (int t = lane + 1; t < total; t++) {
+            if (!((hm >> t) & 1ull)) acc += v_s[wid][t];
+            else break;
+        }
+        int u = __popcll(hm & ((1ull << lane) - 1ull));
+        long long o = poff[i] + u;
+        tcol[o] = colv;
+        tval[o] = acc;
+    }
+    if (lane == 0 && active) uniq[i] = __popcll(hm);
+}
+
+// one workgroup per row; keys/vals/pre point at LDS (medium) or global scratch (large)
+__device__ void spgemm_row_wg(int i, const int* __restrict__ arow, const int* __restrict__ acol, const double* __restrict__ aval,
+                              const int* __restrict__ brow, const int* __restrict__ bcol, const double* __restrict__ bval,
+                              long long o0, int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq,
+                              unsigned long long* keys, double* vals, int* pre, int* lscan) {
+    int tid = threadIdx.x;
+    int s = arow[i], alen = arow[i + 1] - s;
+    // prefix of B-row lengths over the A row (chunks of WG with carry)
+    int carry = 0;
+    for (int base = 0; base < alen; base += WG) {
+        int a = base + tid;
+        int cl = 0;
+        if (a < alen) {
+            int k = acol[s + a];
+            cl = brow[k + 1] - brow[k];
+        }
+        lscan[tid] = cl;
+        __syncthreads();
+        for (int d = 1; d < WG; d <<= 1) {
+            int y = tid >= d ? lscan[tid - d] : 0;
+            __syncthreads();
+            lscan[tid] += y;
+            __syncthreads();
+        }
+        if (a < alen) pre[a + 1] = carry + lscan[tid];
+        int tot = lscan[WG - 1];
+        __syncthreads();
+        carry += tot;
+    }
+    if (tid == 0) pre[0] = 0;
+    __syncthreads();
+    int total = carry;
+    int P = 1;
+    while (P < total) P <<= 1;
+    for (int p = tid; p < P; p += WG) {
+        unsigned long long key = ~0ull;
+        double v = 0.0;
+        if (p < total) {
+            int lo = 0, hi = alen;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (pre[mid] <= p) lo = mid;
+                else hi = mid;
+            }
+            int ka = acol[s + lo];
+            int jb = brow[ka] + (p - pre[lo]);
+            v = aval[s + lo] * bval[jb];
+            key = ((unsigned long long)(unsigned)bcol[jb] << 32) | (unsigned)p;
+        }
+        keys[p] = key;
+        vals[p] = v;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= P; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int p = tid; p < P; p += WG) {
+                int q = p ^ j;
+                if (q > p) {
+                    bool up = (p & kk) == 0;
+                    unsigned long long a = keys[p], b = keys[q];
+                    if ((a > b) == up) {
+                        keys[p] = b;
+                        keys[q] = a;
+                        double t = vals[p];
+                        vals[p] = vals[q];
+                        vals[q] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // heads, their order, sums
+    int ucarry = 0;
+    for (int base = 0; base < total; base += WG) {
+        int p = base + tid;
+        int hd = 0;
+        if (p < total) hd = (p == 0 || (keys[p] >> 32) != (keys[p - 1] >> 32)) ? 1 : 0;
+        lscan[tid] = hd;
+        __syncthreads();
+        for (int d = 1; d < WG; d <<= 1) {
+            int y = tid >= d ? lscan[tid - d] : 0;
+            __syncthreads();
+            lscan[tid] += y;
+            __syncthreads();
+        }
+        if (hd) {
+            unsigned cc = (unsigned)(keys[p] >> 32);
+            double acc = vals[p];
+            for (int t = p + 1; t < total && (unsigned)(keys[t] >> 32) == cc; t++) acc += vals[t];
+            long long o = o0 + ucarry + lscan[tid] - 1;
+            tcol[o] = (int)cc;
+            tval[o] = acc;
+        }
+        int tot = lscan[WG - 1];
+        __syncthreads();
+        ucarry += tot;
+    }
+    if (tid == 0) uniq[i] = ucarry;
+}
+
+__global__ __launch_bounds__(WG) void k_spgemm_med(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
+                                                   const int* __restrict__ acol, const double* __restrict__ aval,
+                                                   const int* __restrict__ brow, const int* __restrict__ bcol,
+                                                   const double* __restrict__ bval, const long long* __restrict__ poff,
+                                                   int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq) {
+    __shared__ unsigned long long keys[MED_P];
+    __shared__ double vals[MED_P];
+    __shared__ int pre[MED_P + 1];
+    __shared__ int lscan[WG];
+    int i = rows[blockIdx.x];
+    spgemm_row_wg(i, arow, acol, aval, brow, bcol, bval, poff[i], tcol, tval, uniq, keys, vals, pre, lscan);
+}
+
+__global__ __launch_bounds__(WG) void k_spgemm_large(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
+                                                     const int* __restrict__ acol, const double* __restrict__ aval,
+                                                     const int* __restrict__ brow, const int* __restrict__ bcol,
+                                                     const double* __restrict__ bval, const long long* __restrict__ poff,
+                                                     int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq,
+                                                     unsigned long long* gkeys, double* gvals, int* gpre,
+                                                     const long long* __restrict__ soff, const long long* __restrict__ aoff) {
+    __shared__ int lscan[WG];
+    int r = blockIdx.x;
+    int i = rows[r];
+    spgemm_row_wg(i, arow, acol, aval, brow, bcol, bval, poff[i], tcol, tval, uniq, gkeys + soff[r], gvals + soff[r],
+                  gpre + aoff[r], lscan);
+}
+
+__global__ void k_spgemm_compact(int n, const long long* __restrict__ poff, const int* __restrict__ crow,
+                                 const int* __restrict__ tcol, const double* __restrict__ tval, int* __restrict__ ccol,
+                                 double* __restrict__ cval) {
+    int i = (blockIdx.x * WG + (int)threadIdx.x) / 16;
+    int lane = (int)threadIdx.x % 16;
+    if (i >= n) return;
+    long long src = poff[i];
+    int dst = crow[i], len = crow[i + 1] - crow[i];
+    for (int t = lane; t < len; t += 16) {
+        ccol[dst + t] = tcol[src + t];
+        cval[dst + t] = tval[src + t];
+    }
+}
+
+// sizes of the large-bin scratch for each large row
+__global__ void k_large_sizes(int nl, const int* __restrict__ rows, const long long* __restrict__ cnt,
+                              const int* __restrict__ arow, long long* __restrict__ psz, long long* __restrict__ asz) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nl) return;
+    int i = rows[r];
+    long long p = cnt[i], P = 1;
+    while (P < p) P <<= 1;
+    psz[r] = P;
+    asz[r] = (long long)(arow[i + 1] - arow[i]) + 1;
+}
+
+void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
+    BCMGX_CHECK(A.m == B.n, BCMGX_ERR_ARG, "spgemm: inner dimensions differ");
+    int n = A.n;
+    C.n = n;
+    C.m = B.m;
+    if (n == 0) {
+        C.alloc(c, 0, B.m, 0);
+        fill<int>(c, C.row.get(), 1, 0);
+        return;
+    }
+    DBuf<long long> cnt(c, n), poff(c, (size_t)n + 1);
+    k_count_products<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
+    BCMGX_LAUNCHED();
+    long long total = scan_exclusive(c, cnt.get(), poff.get(), n);
+    size_t tsz = (size_t)std::max<long long>(total, 1);
+    DBuf<int> tcol(c, tsz);
+    DBuf<double> tval(c, tsz);
+    DBuf<int> uniq(c, (size_t)n + 1);
+    DBuf<int> bins(c, (size_t)3 * n), bincount(c, 3);
+    fill<int>(c, bincount.get(), 3, 0);
+    k_bin_rows<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), cnt.get(), bins.get(), bincount.get(), n);
+    BCMGX_LAUNCHED();
+    int bc[3];
+    c->d2h(bc, bincount.get(), 3);
+    if (bc[0] > 0) {
+        k_spgemm_small<<<grid_for(bc[0], 4), WG, 0, c->stream>>>(bc[0], bins.get(), A.row.get(), A.col.get(), A.val.get(),
+                                                                  B.row.get(), B.col.get(), B.val.get(), poff.get(),
+                                                                  tcol.get(), tval.get(), uniq.get());
+        BCMGX_LAUNCHED();
+    }
+    if (bc[1] > 0) {
+        k_spgemm_med<<<bc[1], WG, 0, c->stream>>>(bc[1], bins.get() + n, A.row.get(), A.col.get(), A.val.get(), B.row.get(),
+                                                  B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get());
+        BCMGX_LAUNCHED();
+    }
+    if (bc[2] > 0) {
+        int nl = bc[2];
+        DBuf<long long> psz(c, nl), asz(c, nl), soff(c, (size_t)nl + 1), aoff(c, (size_t)nl + 1);
+        k_large_sizes<<<grid_for(nl, WG), WG, 0, c->stream>>>(nl, bins.get() + 2 * (size_t)n, cnt.get(), A.row.get(), psz.get(), asz.get());
+        BCMGX_LAUNCHED();
+        long long ptot = scan_exclusive(c, psz.get(), soff.get(), nl);
+        long long atot = scan_exclusive(c, asz.get(), aoff.get(), nl);
+        DBuf<unsigned long long> gk(c, (size_t)ptot);
+        DBuf<double> gv(c, (size_t)ptot);
+        DBuf<int> gp(c, (size_t)atot);
+        k_spgemm_large<<<nl, WG, 0, c->stream>>>(nl, bins.get() + 2 * (size_t)n, A.row.get(), A.col.get(), A.val.get(),
+                                                 B.row.get(), B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(),
+                                                 uniq.get(), gk.get(), gv.get(), gp.get(), soff.get(), aoff.get());
+        BCMGX_LAUNCHED();
+        c->sync();  // keep the scratch alive until the kernel is done
+    }
+    C.row.reset(c, (size_t)n + 1);
+    long long nnz = scan_exclusive(c, uniq.get(), C.row.get(), n);
+    BCMGX_CHECK(nnz < (1ll << 31), BCMGX_ERR_UNSUPPORTED, "spgemm: result has more than 2^31 entries");
+    C.nnz = (int)nnz;
+    C.col.reset(c, (size_t)std::max<long long>(nnz, 1));
+    C.val.reset(c, (size_t)std::max<long long>(nnz, 1));
+    k_spgemm_compact<<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff.get(), C.row.get(), tcol.get(), tval.get(), C.col.get(),
+                                                                C.val.get());
+    BCMGX_LAUNCHED();
+    c->sync();
+}
+
+// ------------------------------------------------------ adjoint gradient ---
+// g[e] = -sum_c lambda[row(e), c] x[col(e), c] on the stored pattern of A
+template <int L>
+__global__ __launch_bounds__(WG) void k_outer_grad(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
+                                                   const double* __restrict__ lam, const double* __restrict__ x,
+                                                   double* __restrict__ g) {
+    int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
+    int lane = (int)threadIdx.x % L;
+    if (i >= n) return;
+    for (int j = row[i] + lane; j < row[i + 1]; j += L) {
+        long long cc = col[j];
+        double s = 0.0;
+        for (int q = 0; q < k; q++) s += lam[(long long)i * k + q] * x[cc * k + q];
+        g[j] = -s;
+    }
+}
+void sparse_outer_grad(Context* c, const DCsr& A, const double* lambda, const double* x, int k, double* g) {
+    if (A.n <= 0) return;
+    k_outer_grad<8><<<grid_for(A.n, WG / 8), WG, 0, c->stream>>>(A.n, k, A.row.get(), A.col.get(), lambda, x, g);
+    BCMGX_LAUNCHED();
+}
+
+}  // namespace bcmgx

@@ -1,0 +1,226 @@
+// spmv.hip -- CSR SpMV / SpMM, l1-Jacobi sweeps and grid transfers.
+//
+// One vector (k == 1): a group of L lanes (L | 64) owns one row and strides
+// over its nonzeros, then reduces with xor-shuffles inside the group -- the
+// wave64 counterpart of the reference's mini-warp kernels
+// (_CSR_vector_mul_mini_warp<OP>, src/datastruct/CSR.cu:452-492;
+// _jacobi_it_full, src/preconditioner/l1jacobi/l1jacobi.cu:344-379).
+// k vectors (k > 1): a group of KP >= k lanes owns one row, lane c handles
+// column c, so every (val, col) pair is loaded once for all k right-hand sides
+// and the x gather of a nonzero is one contiguous k-wide segment.
+// Matrix values and columns are streamed with nontemporal loads so the L2
+// keeps the gathered x; blocks are remapped so each XCD walks one contiguous
+// range of rows.
+#include "kernels.h"
+
+namespace bcmgx {
+
+template <typename T, typename V, int L, SpOp OP>
+__global__ __launch_bounds__(WG) void k_spmv1(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                              const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
+                                              const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
+                                              const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    constexpr int RPB = WG / L;
+    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int r = blk * RPB + (int)threadIdx.x / L;
+    int lane = (int)threadIdx.x % L;
+    if (r >= n) return;  // a whole L-lane group leaves together
+    int s = row[r], e = row[r + 1];
+    V t = 0;
+    for (int j = s + lane; j < e; j += L) {
+        V a = (V)ldg_nt(val + j);
+        int cc = ldg_nt(col + j);
+        if (OP == SpOp::AXPBY) t += (alpha * a) * x[cc];
+        else t += a * x[cc];
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, L);
+    if (lane == 0) {
+        if (OP == SpOp::AX) y[r] = t;
+        else if (OP == SpOp::RES) y[r] = b[r] - t;
+        else if (OP == SpOp::AXPBY) y[r] = t + beta * y[r];
+        else y[r] = ((-t + b[r]) / D[r]) + x[r];
+    }
+}
+
+template <typename T, typename V, int KP, SpOp OP>
+__global__ __launch_bounds__(WG) void k_spmmk(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
+                                              const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
+                                              const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
+                                              const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    constexpr int RPB = WG / KP;
+    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int r = blk * RPB + (int)threadIdx.x / KP;
+    int c = (int)threadIdx.x % KP;
+    if (r >= n || c >= k) return;
+    int s = row[r], e = row[r + 1];
+    V t = 0;
+#pragma unroll 4
+    for (int j = s; j < e; j++) {
+        V a = (V)val[j];
+        long long cc = col[j];
+        if (OP == SpOp::AXPBY) t += (alpha * a) * x[cc * k + c];
+        else t += a * x[cc * k + c];
+    }
+    long long o = (long long)r * k + c;
+    if (OP == SpOp::AX) y[o] = t;
+    else if (OP == SpOp::RES) y[o] = b[o] - t;
+    else if (OP == SpOp::AXPBY) y[o] = t + beta * y[o];
+    else y[o] = ((-t + b[o]) / D[r]) + x[o];
+}
+
+int default_lanes(long long nnz, int n) {
+    if (n <= 0) return 1;
+    double d = (double)nnz / n;
+    if (d <= 1.5) return 1;
+    if (d <= 3) return 2;
+    if (d <= 6) return 4;
+    if (d <= 12) return 8;
+    if (d <= 24) return 16;
+    if (d <= 48) return 32;
+    return 64;
+}
+
+template <typename T, typename V, SpOp OP>
+static void launch1(Context* c, const SpmvArgs& a, int L) {
+    const T* val = (const T*)a.val;
+    const V* x = (const V*)a.x;
+    V* y = (V*)a.y;
+    const V* b = (const V*)a.b;
+    const V* D = (const V*)a.D;
+    V al = (V)a.alpha, be = (V)a.beta;
+#define BCMGX_L(LL)                                                                                            \
+    case LL:                                                                                                   \
+        k_spmv1<T, V, LL, OP><<<grid_for(a.n, WG / LL), WG, 0, c->stream>>>(a.n, a.row, a.col, val, x, y, b, D, \
+                                                                          al, be, a.stop);                   \
+        break;
+    switch (L) {
+        BCMGX_L(1) BCMGX_L(2) BCMGX_L(4) BCMGX_L(8) BCMGX_L(16) BCMGX_L(32) BCMGX_L(64)
+    default: throw Error(BCMGX_ERR_ARG, "lanes per row must be a power of two <= 64");
+    }
+#undef BCMGX_L
+    BCMGX_LAUNCHED();
+}
+
+template <typename T, typename V, SpOp OP>
+static void launchk(Context* c, const SpmvArgs& a) {
+    const T* val = (const T*)a.val;
+    const V* x = (const V*)a.x;
+    V* y = (V*)a.y;
+    const V* b = (const V*)a.b;
+    const V* D = (const V*)a.D;
+    V al = (V)a.alpha, be = (V)a.beta;
+    int KP = 2;
+    while (KP < a.k) KP <<= 1;
+#define BCMGX_K(KK)                                                                                                  \
+    case KK:                                                                                                         \
+        k_spmmk<T, V, KK, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, x, y, b, D, \
+                                                                            al, be, a.stop);                       \
+        break;
+    switch (KP) {
+        BCMGX_K(2) BCMGX_K(4) BCMGX_K(8) BCMGX_K(16) BCMGX_K(32) BCMGX_K(64)
+    default: throw Error(BCMGX_ERR_ARG, "at most 64 right-hand sides");
+    }
+#undef BCMGX_K
+    BCMGX_LAUNCHED();
+}
+
+template <typename T, typename V>
+void spmv(Context* c, SpOp op, const SpmvArgs& a) {
+    if (a.n <= 0) return;
+    if (a.k == 1) {
+        int L = a.lanes > 0 ? a.lanes : default_lanes(a.nnz, a.n);
+        switch (op) {
+        case SpOp::AX: launch1<T, V, SpOp::AX>(c, a, L); break;
+        case SpOp::RES: launch1<T, V, SpOp::RES>(c, a, L); break;
+        case SpOp::AXPBY: launch1<T, V, SpOp::AXPBY>(c, a, L); break;
+        case SpOp::JACOBI: launch1<T, V, SpOp::JACOBI>(c, a, L); break;
+        }
+    } else {
+        switch (op) {
+        case SpOp::AX: launchk<T, V, SpOp::AX>(c, a); break;
+        case SpOp::RES: launchk<T, V, SpOp::RES>(c, a); break;
+        case SpOp::AXPBY: launchk<T, V, SpOp::AXPBY>(c, a); break;
+        case SpOp::JACOBI: launchk<T, V, SpOp::JACOBI>(c, a); break;
+        }
+    }
+}
+template void spmv<double, double>(Context*, SpOp, const SpmvArgs&);
+template void spmv<float, float>(Context*, SpOp, const SpmvArgs&);
+
+// ---------------------------------------------------------------- vectors ---
+template <typename V>
+__global__ void k_jacobi0(long long nk, int k, const V* __restrict__ f, const V* __restrict__ D, V* __restrict__ x,
+                          const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    // ((-(A 0) + f) / D) + 0 == f / D
+    x[t] = f[t] / D[t / k];
+}
+template <typename V>
+void jacobi_from_zero(Context* c, int n, int k, const V* f, const V* D, V* x, const int* stop) {
+    long long nk = (long long)n * k;
+    if (!nk) return;
+    k_jacobi0<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, f, D, x, stop);
+    BCMGX_LAUNCHED();
+}
+template void jacobi_from_zero<double>(Context*, int, int, const double*, const double*, double*, const int*);
+template void jacobi_from_zero<float>(Context*, int, int, const float*, const float*, float*, const int*);
+
+__global__ void k_jacobi0_f64in(long long nk, int k, const double* __restrict__ r, const float* __restrict__ D,
+                                float* __restrict__ F, float* __restrict__ X, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    float f = (float)r[t];
+    F[t] = f;
+    X[t] = f / D[t / k];
+}
+void jacobi_from_zero_f64in(Context* c, int n, int k, const double* r, const float* D, float* F, float* X, const int* stop) {
+    long long nk = (long long)n * k;
+    if (!nk) return;
+    k_jacobi0_f64in<<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, r, D, F, X, stop);
+    BCMGX_LAUNCHED();
+}
+
+template <typename V>
+__global__ void k_prolong(long long nk, int k, const int* __restrict__ agg, const V* __restrict__ pval,
+                          const V* __restrict__ xc, V* __restrict__ x, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    long long i = t / k;
+    int cc = (int)(t - i * k);
+    V tt = pval[i] * xc[(long long)agg[i] * k + cc];
+    x[t] = tt + x[t];
+}
+template <typename V>
+void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const V* xc, V* x, const int* stop) {
+    long long nk = (long long)n * k;
+    if (!nk) return;
+    k_prolong<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, agg, pval, xc, x, stop);
+    BCMGX_LAUNCHED();
+}
+template void prolong_add<double>(Context*, int, int, const int*, const double*, const double*, double*, const int*);
+template void prolong_add<float>(Context*, int, int, const int*, const float*, const float*, float*, const int*);
+
+template <typename A, typename B>
+__global__ void k_convert_stop(const A* __restrict__ s, B* __restrict__ d, long long n, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) d[t] = (B)s[t];
+}
+template <typename A, typename B>
+void convert_stop(Context* c, const A* s, B* d, long long n, const int* stop) {
+    if (!n) return;
+    k_convert_stop<A, B><<<grid_for(n, WG), WG, 0, c->stream>>>(s, d, n, stop);
+    BCMGX_LAUNCHED();
+}
+template void convert_stop<float, double>(Context*, const float*, double*, long long, const int*);
+template void convert_stop<double, float>(Context*, const double*, float*, long long, const int*);
+template void convert_stop<double, double>(Context*, const double*, double*, long long, const int*);
+
+}  // namespace bcmgx

@@ -141,6 +141,15 @@ int bcmgx_gen_diffusion7p(int nx, int ny, int nz, int kind, double p1, double p2
 int bcmgx_read_mtx(const char* path, int* n, int* m, int* nnz, int** row, int** col, double** val);
 void bcmgx_free(void* p);
 
+/* ---- device vectors ------------------------------------------------------
+ * Device buffers on the context's device (Vector::init<T>(n, true, true) and
+ * Vector::copyTo / copyToDevice, src/datastruct/vector.h) so callers without
+ * their own allocator can keep inputs resident in HBM.  kind of bcmgx_memcpy:
+ * 0 host->device, 1 device->host, 2 device->device; it completes before returning. */
+int bcmgx_device_alloc(bcmgx_context ctx, size_t bytes, void** ptr);
+int bcmgx_device_free(bcmgx_context ctx, void* ptr);
+int bcmgx_memcpy(bcmgx_context ctx, void* dst, const void* src, size_t bytes, int kind);
+
 /* ---- preconditioner -----------------------------------------------------
  * prec_setup (src/preconditioner/prec_setup.h:55): the preconditioner type
  * comes from params->preconditioner (the reference's cgsprec::ptype).
@@ -215,6 +224,11 @@ int bcmgx_matching(bcmgx_context ctx, bcmgx_matrix A, const double* w, int* M);
  * agg[i] coarse index, pval[i] value; returns the coarse size in *nc. */
 int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double* w, int* agg,
                            double* pval, int* nc);
+/* Micro-benchmark of one SpMV/SpMM launch (driverSpMV, src/example/driverSpMV.cu):
+ * y = A x on k interleaved columns, FP64 (precision 0) or FP32 matrix and
+ * vectors (precision 1); `reps` timed launches on the context stream after one
+ * warm-up, bracketed by HIP events.  *ms = average milliseconds per launch. */
+int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms);
 /* Sustained device-to-device copy bandwidth (GB/s) over `bytes` bytes. */
 int bcmgx_stream_copy_gbs(bcmgx_context ctx, size_t bytes, int reps, double* gbs);
 

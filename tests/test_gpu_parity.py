@@ -1,0 +1,340 @@
+"""HIP path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Integer/index results (matching, aggregates, sparsity patterns) must be
+bit-identical; setup values (matching weights, prolongator values, Galerkin
+coarse operators) are computed in the oracle's operation order without FMA
+contraction and are checked bit-exactly too (the stated bar is 1e-12
+relative); solve-path values are checked to 1e-12/1e-10 relative and FCG
+iteration counts to +-1."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import bootcmatchgx_amd as bx
+import oracle_py as O
+from bootcmatchgx_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, ocsr):
+    n, m, _ = ocsr.dims()
+    r, c, v = ocsr.arrays()
+    return bx.CSR(ctx, n, m, r, c, v)
+
+
+def rand_spd(n, density, seed, ties=False):
+    S = sp.random(n, n, density=density, random_state=seed)
+    S = S + S.T
+    S.data[:] = -(np.round(S.data * 3) / 3 if ties else S.data)
+    S = S + sp.diags(np.asarray(abs(S).sum(1)).ravel() + 1.0)
+    S = S.tocsr()
+    S.sort_indices()
+    return S
+
+
+CASES = {
+    "lap7_12": lambda: O.laplacian3d(7, 12, 12, 12),
+    "lap7_9x7x5": lambda: O.laplacian3d(7, 9, 7, 5),
+    "lap27_10": lambda: O.laplacian3d(27, 10, 10, 10),
+    "lap7_block": lambda: O.laplacian3d(7, 2, 2, 5, 3, 4, 2),
+    "diff_exp": lambda: O.diffusion7p(11, 10, 9, 0, 2.0, 0.0, 2),
+    "diff_unif": lambda: O.diffusion7p(11, 10, 9, 1, 0.5, 2.0, 4),
+    "rand_ties": lambda: O.Csr.from_scipy(rand_spd(500, 0.02, 5, ties=True)),
+    "rand": lambda: O.Csr.from_scipy(rand_spd(500, 0.02, 6)),
+}
+
+
+def params(prec, **kw):
+    p = bx.Params(solver="FCG", preconditioner=prec, dispnorm=0)
+    for k, v in kw.items():
+        p.set(k, v)
+    return p
+
+
+def oparams(p):
+    q = abi.Params()
+    C = __import__("ctypes")
+    C.pointer(q)[0] = p.s
+    return q
+
+
+# ------------------------------------------------------------------- SpMV
+@pytest.mark.parametrize("name", ["lap7_12", "lap27_10", "rand"])
+@pytest.mark.parametrize("lanes", [0, 1, 4, 16, 64])
+def test_spmv_vs_oracle(ctx, name, lanes):
+    oa = CASES[name]()
+    A = dev(ctx, oa)
+    rng = np.random.default_rng(3)
+    x, y = rng.normal(size=A.m), rng.normal(size=A.n)
+    for op, al, be in [(1, 1.0, 0.0), (2, -1.0, 1.0), (0, 0.7, -1.3)]:
+        got = A.spmv(x, y, al, be, op, lanes)
+        ref = O.spmv(oa, x, y, al, be, op)
+        scale = np.abs(O.spmv(O.Csr.from_scipy(abs(oa.to_scipy())), np.abs(x))) + np.abs(y)
+        assert np.all(np.abs(got - ref) <= 1e-14 * scale + 1e-300), (op, lanes)
+
+
+# --------------------------------------------------------------- matching
+@pytest.mark.parametrize("name", list(CASES))
+def test_matching_bit_exact(ctx, name):
+    oa = CASES[name]()
+    A = dev(ctx, oa)
+    n = A.n
+    w = np.random.default_rng(1).uniform(0.5, 2.0, n) if name.startswith("rand") else np.ones(n)
+    assert np.array_equal(A.matching(w), O.suitor(oa, w))
+
+
+def test_matching_bcsstk09(ctx, golden):
+    oa = O.read_mtx(os.path.join(golden, "bcsstk09.mtx"))
+    A = dev(ctx, oa)
+    w = np.ones(A.n)
+    assert np.array_equal(A.matching(w), O.suitor(oa, w))
+
+
+def test_make_prolongator_exact(ctx):
+    for name in ["lap7_9x7x5", "rand_ties", "diff_exp"]:
+        oa = CASES[name]()
+        n = oa.dims()[0]
+        w = np.random.default_rng(2).uniform(-1, 2, n)
+        w[::17] = 0.0  # bad singletons
+        M = O.suitor(oa, np.abs(w) + 0.5)
+        a1, p1, c1 = bx.make_prolongator(ctx, M, w)
+        a2, p2, c2 = O.make_p(M, w)
+        assert c1 == c2 and np.array_equal(a1, a2) and np.array_equal(p1, p2)
+
+
+# ------------------------------------------------------------------ SpGEMM
+def _assert_same_csr(Cg, Co):
+    r1, c1, v1 = Cg.to_host()
+    r2, c2, v2 = Co.arrays()
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+    assert np.array_equal(v1, v2), np.max(np.abs(v1 - v2) / (np.abs(v2) + 1e-300))
+
+
+@pytest.mark.parametrize("name", ["lap7_12", "lap27_10", "rand", "diff_exp"])
+def test_spgemm_galerkin_bit_exact(ctx, name):
+    oa = CASES[name]()
+    n = oa.dims()[0]
+    w = np.ones(n)
+    M = O.suitor(oa, w)
+    agg, pv, nc = O.make_p(M, w)
+    P = O.Csr.from_arrays(n, nc, np.arange(n + 1), agg, pv)
+    R = O.transpose(P)
+    A, Pd, Rd = dev(ctx, oa), dev(ctx, P), dev(ctx, R)
+    AP = A.spgemm(Pd)
+    _assert_same_csr(AP, O.spgemm(oa, P))
+    _assert_same_csr(Rd.spgemm(AP), O.spgemm(R, O.spgemm(oa, P)))
+    _assert_same_csr(Pd.transpose(), R)
+
+
+def test_spgemm_medium_and_large_rows(ctx):
+    # A*A of a 27-point stencil: up to 729 products per row (LDS bin)
+    oa = O.laplacian3d(27, 9, 8, 7)
+    A = dev(ctx, oa)
+    _assert_same_csr(A.spgemm(A), O.spgemm(oa, oa))
+    # a few dense rows: > 2048 products per row (global-memory bin)
+    rng = np.random.default_rng(4)
+    S = sp.random(300, 300, density=0.02, random_state=7, format="lil")
+    S[5, :] = rng.normal(size=300)
+    S[17, :] = rng.normal(size=300)
+    S = S.tocsr()
+    S.sort_indices()
+    T = sp.random(300, 200, density=0.05, random_state=8, format="lil")
+    T[:, 3] = rng.normal(size=(300, 1))
+    T = T.tocsr()
+    T.sort_indices()
+    oS, oT = O.Csr.from_scipy(S), O.Csr.from_scipy(T)
+    _assert_same_csr(dev(ctx, oS).spgemm(dev(ctx, oT)), O.spgemm(oS, oT))
+    # empty rows and an empty product
+    E = O.Csr.from_arrays(4, 4, np.array([0, 0, 1, 1, 1]), np.array([2]), np.array([3.0]))
+    _assert_same_csr(dev(ctx, E).spgemm(dev(ctx, E)), O.spgemm(E, E))
+
+
+# --------------------------------------------------------------- hierarchy
+HIER = [
+    ("lap7_24", lambda: O.laplacian3d(7, 24, 24, 24), {}),
+    ("lap27_18", lambda: O.laplacian3d(27, 18, 18, 18), {}),
+    ("lap7_odd", lambda: O.laplacian3d(7, 17, 13, 11), {"max_coarse_size": 20}),
+    ("diff_exp", lambda: O.diffusion7p(20, 18, 16, 0, 2.0, 0.0, 2), {"max_coarse_size": 50}),
+    ("diff_unif", lambda: O.diffusion7p(20, 20, 20, 1, 0.5, 2.0, 4), {}),
+    ("one_sweep", lambda: O.laplacian3d(7, 16, 16, 16), {"aggrsweeps": 0, "max_coarse_size": 30}),
+]
+
+
+@pytest.mark.parametrize("name,make,kw", HIER)
+def test_hierarchy_bit_exact(ctx, name, make, kw):
+    oa = make()
+    A = dev(ctx, oa)
+    p = params("BCMG", **kw)
+    P = bx.prec_setup(ctx, A, p)
+    OP = O.Prec(oa, oparams(p))
+    assert P.levels() == OP.levels()
+    for l in range(P.levels()):
+        r1 = P.level_matrix(l)
+        r2 = OP.level_A(l).to_scipy()
+        assert np.array_equal(r1.indptr, r2.indptr) and np.array_equal(r1.indices, r2.indices)
+        assert np.array_equal(r1.data, r2.data), (l, np.max(np.abs(r1.data - r2.data) / np.abs(r2.data)))
+        if l < P.levels() - 1:
+            a1, p1 = P.level_prolongator(l)
+            a2, p2 = OP.level_P(l)
+            assert np.array_equal(a1, a2) and np.array_equal(p1, p2)
+    info = P.info()
+    st = OP.stats()
+    assert info["max_coarse_size"] == st["max_coarse_size"]
+    assert abs(info["op_cmplx"] - st["op_cmplx"]) < 1e-12
+    assert abs(info["avg_cratio"] - st["avg_cratio"]) < 1e-12
+
+
+# ------------------------------------------------------------- whole solve
+@pytest.mark.parametrize("pts", [7, 27])
+@pytest.mark.parametrize("prec", ["BCMG", "AFSAI", "L1_JACOBI", "NONE"])
+def test_solve_reference_fixture(ctx, golden, pts, prec):
+    """launchAndCompare.mk criterion on the reference's 10^3 solutions."""
+    oa = O.laplacian3d(pts, 10, 10, 10)
+    A = dev(ctx, oa)
+    p = params(prec)
+    P = bx.prec_setup(ctx, A, p)
+    x, out = bx.solve(ctx, A, np.ones(1000), np.zeros(1000), p, P)
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_1x1x1_sol.txt"))
+    assert out["retv"] == 0
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+    _, oi = O.fcg(oa, O.Prec(oa, oparams(p)), oparams(p), np.ones(1000))
+    assert abs(out["niter"] - oi["niter"]) <= 1
+
+
+@pytest.mark.parametrize("prec", ["BCMG", "AFSAI", "L1_JACOBI"])
+def test_solve_bcsstk09(ctx, golden, prec):
+    oa = O.read_mtx(os.path.join(golden, "bcsstk09.mtx"))
+    A = dev(ctx, oa)
+    p = params(prec, itnlim=5000)
+    x, out = bx.solve(ctx, A, np.ones(A.n), None, p, bx.prec_setup(ctx, A, p))
+    ref = np.loadtxt(os.path.join(golden, "bcsstk09_sol.txt"))
+    assert out["retv"] == 0 and np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+def test_c0_iteration_count_and_residual(ctx):
+    """Config C0: 7-point 50^3, b = A x*, rtol 1e-8, 1/1 sweeps."""
+    oa = O.laplacian3d(7, 50, 50, 50)
+    n = 125000
+    xs = np.random.default_rng(0).uniform(-1, 1, n)
+    b = O.spmv(oa, xs)
+    A = dev(ctx, oa)
+    p = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1)
+    P = bx.prec_setup(ctx, A, p)
+    x, out = bx.solve(ctx, A, b, None, p, P)
+    _, oi = O.fcg(oa, O.Prec(oa, oparams(p)), oparams(p), b)
+    assert abs(out["niter"] - oi["niter"]) <= 1
+    r = b - O.spmv(oa, x)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.0000001
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+
+
+def test_prec_apply_matches_oracle(ctx):
+    oa = O.laplacian3d(27, 14, 14, 14)
+    A = dev(ctx, oa)
+    r = np.random.default_rng(5).normal(size=A.n)
+    for prec in ["BCMG", "AFSAI", "L1_JACOBI", "NONE"]:
+        p = params(prec, prerelax_sweeps=2, postrelax_sweeps=1)
+        got = bx.prec_setup(ctx, A, p).apply(r)
+        ref = O.Prec(oa, oparams(p)).apply(r)
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), prec
+
+
+@pytest.mark.parametrize("cycle", ["W_CYCLE", "H_CYCLE", "VARIABLE_V_CYCLE"])
+def test_other_cycles_match_oracle(ctx, cycle):
+    oa = O.laplacian3d(7, 20, 20, 20)
+    A = dev(ctx, oa)
+    p = params("BCMG", cycle_type=cycle, max_coarse_size=40, aggrsweeps=0)
+    r = np.random.default_rng(6).normal(size=A.n)
+    got = bx.prec_setup(ctx, A, p).apply(r)
+    ref = O.Prec(oa, oparams(p)).apply(r)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_afsai_factor_vs_oracle(ctx):
+    for oa in [O.laplacian3d(7, 9, 8, 7), O.laplacian3d(27, 7, 7, 7), O.diffusion7p(8, 8, 8, 1, 0.5, 2.0, 4)]:
+        A = dev(ctx, oa)
+        p = params("AFSAI")
+        G = bx.prec_setup(ctx, A, p).afsai_factor()
+        Go = O.Prec(oa, oparams(p)).afsai_G().to_scipy()
+        assert np.array_equal(G.indptr, Go.indptr) and np.array_equal(G.indices, Go.indices)
+        assert np.max(np.abs(G.data - Go.data) / np.abs(Go.data)) <= 1e-12
+
+
+# ------------------------------------------------------------- extensions
+def test_fp32_hierarchy_converges_like_fp64(ctx):
+    oa = O.laplacian3d(27, 24, 24, 24)
+    n = oa.dims()[0]
+    xs = np.random.default_rng(1).uniform(-1, 1, n)
+    b = O.spmv(oa, xs)
+    A = dev(ctx, oa)
+    p64 = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1)
+    p32 = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, precision="FP32")
+    x64, o64 = bx.solve(ctx, A, b, None, p64, bx.prec_setup(ctx, A, p64))
+    x32, o32 = bx.solve(ctx, A, b, None, p32, bx.prec_setup(ctx, A, p32))
+    _, oi = O.fcg(oa, O.Prec(oa, oparams(p32)), oparams(p32), b)
+    assert o32["retv"] == 0 and abs(o32["niter"] - o64["niter"]) <= 1 and abs(o32["niter"] - oi["niter"]) <= 1
+    r = b - O.spmv(oa, x32)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.0000001
+    assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_batched_columns_match_single_solves(ctx, precision):
+    oa = O.laplacian3d(7, 30, 30, 30)
+    n = oa.dims()[0]
+    k = 5
+    B = np.random.default_rng(3).normal(size=(n, k))
+    B[:, 4] *= 1e-3  # different scales converge independently
+    A = dev(ctx, oa)
+    p = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, precision=precision)
+    P = bx.prec_setup(ctx, A, p)
+    X, out = bx.solve(ctx, A, B, None, p, P)
+    for j in range(k):
+        xj, oj = bx.solve(ctx, A, B[:, j], None, p, P)
+        assert abs(out["niter_cols"][j] - oj["niter"]) <= 1
+        assert np.linalg.norm(X[:, j] - xj) <= 1e-6 * np.linalg.norm(xj)
+        r = B[:, j] - O.spmv(oa, X[:, j])
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, j]) * 1.0000001
+
+
+def test_graph_replay_equals_eager(ctx):
+    oa = O.laplacian3d(27, 16, 16, 16)
+    b = O.spmv(oa, np.random.default_rng(9).uniform(-1, 1, 4096))
+    A = dev(ctx, oa)
+    res = []
+    for g in (1, 0, 1):
+        p = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, use_graph=g)
+        P = bx.prec_setup(ctx, A, p)
+        x1, o1 = bx.solve(ctx, A, b, None, p, P)
+        x2, o2 = bx.solve(ctx, A, b, None, p, P)  # the graph is replayed for a second solve
+        assert o1["niter"] == o2["niter"] and np.array_equal(x1, x2)
+        res.append((x1, o1["niter"]))
+    assert res[0][1] == res[1][1] == res[2][1]
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][0], res[2][0])
+
+
+def test_itnlim_and_info_file(ctx, tmp_path):
+    oa = O.laplacian3d(7, 12, 12, 12)
+    A = dev(ctx, oa)
+    p = params("BCMG", rtol=1e-30, itnlim=5)
+    P = bx.prec_setup(ctx, A, p)
+    x, out = bx.solve(ctx, A, np.ones(A.n), None, p, P)
+    assert out["retv"] == -1 and out["niter"] == 6
+    f = tmp_path / "info.txt"
+    bx.dump_info(str(f), p, P, out)
+    txt = f.read_text()
+    assert "*** Exceeded max iter ***" in txt and "Total iterations" in txt and "Number of levels" in txt
+
+
+def test_sparse_outer_grad(ctx):
+    oa = O.laplacian3d(7, 6, 5, 4)
+    A = dev(ctx, oa)
+    rng = np.random.default_rng(2)
+    lam, x = rng.normal(size=(A.n, 3)), rng.normal(size=(A.n, 3))
+    g = A.sparse_outer_grad(lam, x)
+    r, c, _ = oa.arrays()
+    rows = np.repeat(np.arange(A.n), np.diff(r))
+    assert np.allclose(g, -np.sum(lam[rows] * x[c], axis=1), rtol=1e-14, atol=1e-14)

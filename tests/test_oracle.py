@@ -1,0 +1,286 @@
+"""The CPU oracle pinned against the reference's own fixtures, closed forms,
+brute force on tiny inputs and independent re-computations (no GPU)."""
+import os
+
+import numpy as np
+import pytest
+import scipy.io
+import scipy.sparse as sp
+
+import oracle_py as O
+from bootcmatchgx_amd import abi
+
+
+def fcg_params(prec, **kw):
+    p = abi.reference_defaults()
+    p.solver = abi.SOLVER["FCG"]
+    p.preconditioner = abi.PREC[prec]
+    p.dispnorm = 0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def mtx(path):
+    M = scipy.io.mmread(path).tocsr()
+    M.sort_indices()
+    return M
+
+
+# ---------------------------------------------------------------- generators
+@pytest.mark.parametrize("fname,args", [
+    ("lap3d_7p_2x2x5_3x4x2.mtx", (7, 2, 2, 5, 3, 4, 2)),
+    ("lap3d_27p_5x5x5_1x2x2.mtx", (27, 5, 5, 5, 1, 2, 2)),
+])
+def test_generator_matches_matlab_fixture(golden, fname, args):
+    """laplacian.cu block generators vs the reference's MATLAB-made matrices."""
+    ref = mtx(os.path.join(golden, fname))
+    A = O.laplacian3d(*args).to_scipy()
+    assert A.shape == ref.shape
+    assert (A != ref).nnz == 0
+
+
+@pytest.mark.parametrize("m", [2, 3, 7, 10])
+def test_generator_closed_forms(m):
+    A7 = O.laplacian3d(7, m, m, m).to_scipy()
+    A27 = O.laplacian3d(27, m, m, m).to_scipy()
+    assert A7.nnz == m ** 3 + 6 * (m - 1) * m * m
+    assert A27.nnz == (3 * m - 2) ** 3
+    assert np.all(A7.diagonal() == 6) and np.all(A27.diagonal() == 26)
+    for A in (A7, A27):
+        assert (A - A.T).nnz == 0
+        rs = np.asarray(A.sum(1)).ravel()
+        assert rs.min() >= 0  # interior rows sum to 0, boundary rows > 0
+    # interior row sums are exactly zero
+    idx = (m // 2) * m * m + (m // 2) * m + m // 2
+    if m >= 3:
+        assert np.asarray(A7.sum(1)).ravel()[idx] == 0 and np.asarray(A27.sum(1)).ravel()[idx] == 0
+
+
+def test_diffusion_generator_properties():
+    A = O.diffusion7p(6, 5, 4, 1, 0.5, 2.0, 4).to_scipy()
+    assert A.shape == (120, 120) and (A - A.T).nnz == 0
+    off = A - sp.diags(A.diagonal())
+    assert off.data.max() < 0 and -off.data.min() <= 2.0 and -off.data.max() >= 0.5
+    # diagonal = sum of the six face conductivities: equal to |off-diagonal row
+    # sum| inside the box, larger on rows that touch a Dirichlet face
+    offsum = -np.asarray(off.sum(1)).ravel()
+    assert np.all(A.diagonal() >= offsum * (1 - 1e-15))
+    assert np.isclose(A.diagonal()[1 * 30 + 1 * 6 + 1], offsum[1 * 30 + 1 * 6 + 1], rtol=1e-14)
+    B = O.diffusion7p(6, 5, 4, 1, 0.5, 2.0, 4).to_scipy()
+    assert (A != B).nnz == 0  # seeded, reproducible
+
+
+def test_mtx_reader_symmetric_expansion(golden):
+    A = O.read_mtx(os.path.join(golden, "bcsstk09.mtx")).to_scipy()
+    ref = mtx(os.path.join(golden, "bcsstk09.mtx"))
+    assert abs(A - ref).max() == 0
+
+
+# ------------------------------------------------------------ sparse kernels
+def rand_spd(n, density, seed, ties=False):
+    S = sp.random(n, n, density=density, random_state=seed)
+    S = S + S.T
+    S.data[:] = -(np.round(S.data * 3) / 3 if ties else S.data)
+    S = S + sp.diags(np.asarray(abs(S).sum(1)).ravel() + 1.0)
+    S = S.tocsr()
+    S.sort_indices()
+    return S
+
+
+def test_spmv_ops_vs_dense():
+    S = rand_spd(60, 0.1, 1)
+    A = O.Csr.from_scipy(S)
+    rng = np.random.default_rng(0)
+    x, y = rng.normal(size=60), rng.normal(size=60)
+    D = S.toarray()
+    assert np.allclose(O.spmv(A, x, op=1), D @ x, rtol=1e-14, atol=1e-13)
+    assert np.allclose(O.spmv(A, x, y, op=2), y - D @ x, rtol=1e-14, atol=1e-13)
+    assert np.allclose(O.spmv(A, x, y, alpha=0.5, beta=-2.0, op=0), 0.5 * D @ x - 2 * y, rtol=1e-14, atol=1e-13)
+
+
+def test_spgemm_and_transpose_vs_scipy():
+    S1, S2 = rand_spd(80, 0.05, 2), sp.random(80, 50, density=0.1, random_state=3, format="csr")
+    S2.sort_indices()
+    A, B = O.Csr.from_scipy(S1), O.Csr.from_scipy(S2)
+    C = O.spgemm(A, B).to_scipy()
+    ref = (S1 @ S2).tocsr()
+    ref.sort_indices()
+    assert abs(C - ref).max() < 1e-13
+    T = O.transpose(B).to_scipy()
+    assert abs(T - S2.T).max() == 0
+    assert abs(O.transpose(O.transpose(B)).to_scipy() - S2).max() == 0
+
+
+# ----------------------------------------------------------------- matching
+MATCH_CASES = [
+    ("lap7_10", lambda: O.laplacian3d(7, 10, 10, 10)),
+    ("lap7_9x7x5", lambda: O.laplacian3d(7, 9, 7, 5)),
+    ("lap27_8", lambda: O.laplacian3d(27, 8, 8, 8)),
+    ("diff_exp", lambda: O.diffusion7p(9, 8, 7, 0, 2.0, 0.0, 2)),
+    ("diff_unif", lambda: O.diffusion7p(9, 8, 7, 1, 0.5, 2.0, 4)),
+    ("rand_ties", lambda: O.Csr.from_scipy(rand_spd(300, 0.03, 5, ties=True))),
+    ("rand", lambda: O.Csr.from_scipy(rand_spd(300, 0.03, 6))),
+]
+
+
+@pytest.mark.parametrize("name,make", MATCH_CASES)
+def test_suitor_equals_greedy_and_invariants(name, make):
+    """Sequential Suitor (the GPU semantics) == greedy matching on (W, max id, min id)."""
+    A = make()
+    n, _, _ = A.dims()
+    w = np.random.default_rng(1).uniform(0.5, 2.0, n) if name.startswith("rand") else np.ones(n)
+    M = O.suitor(A, w)
+    G = O.greedy_matching(A, w)
+    assert np.array_equal(M, G)
+    # involution
+    m = M >= 0
+    assert np.all(M[M[m]] == np.nonzero(m)[0])
+    # maximality over positive edges between unmatched vertices
+    r, c, v = O.matching_weights(A, w)
+    for i in np.nonzero(~m)[0]:
+        for j in range(r[i], r[i + 1]):
+            if v[j] > 0:
+                assert M[c[j]] >= 0, "positive edge between two unmatched vertices"
+
+
+def test_matching_weights_symmetric_and_formula():
+    S = rand_spd(40, 0.15, 7)
+    A = O.Csr.from_scipy(S)
+    w = np.random.default_rng(3).uniform(0.5, 2, 40)
+    r, c, v = O.matching_weights(A, w)
+    W = sp.csr_matrix((v, c, r), shape=(40, 40))
+    assert abs(W - W.T).max() == 0
+    D = S.diagonal()
+    Cc = D * w * w
+    i, j = 3, None
+    row = S.getrow(i)
+    for jj, a in zip(row.indices, row.data):
+        if jj == i:
+            continue
+        ah = 1 - 2 * a * w[i] * w[jj] / (Cc[i] + Cc[jj])
+        assert abs(W[i, jj] - np.log(abs(ah))) < 1e-13
+
+
+def test_tied_laplacian_pairs_along_x():
+    """All weights tie on the constant-coefficient 7-point Laplacian: the greedy
+    order (W, max id, min id) pairs every even x index with the next one."""
+    M = O.suitor(O.laplacian3d(7, 10, 6, 4), np.ones(240))
+    assert np.array_equal(M, np.arange(240) ^ 1)
+
+
+def test_make_p_canonical_order():
+    M = np.array([1, 0, -1, 5, -1, 3, -1], np.int32)
+    w = np.array([1.0, 2.0, 3.0, 1.0, 0.0, 1.0, -2.0])
+    agg, pv, nc = O.make_p(M, w)
+    # pairs by master (0,1) -> 0, (3,5) -> 1; good singletons ascending: 2 -> 2, 6 -> 3
+    assert nc == 4
+    assert agg.tolist() == [0, 0, 2, 1, 2, 1, 3]  # bad singleton 4 -> (running count 3) - 1
+    assert np.allclose(pv, [1 / np.sqrt(5), 2 / np.sqrt(5), 1.0, 1 / np.sqrt(2), 0.0, 1 / np.sqrt(2), -1.0])
+
+
+# ---------------------------------------------------------------- hierarchy
+@pytest.mark.parametrize("N,expect", [(125000, 1999), (2097152, 5119), (4096000, 6399), (64000, 1599), (1000, 399)])
+def test_reference_coarse_size_truncation(N, expect):
+    assert int(40 * np.power(float(N), 1.0 / 3.0)) == expect
+
+
+def test_galerkin_matches_dense_triple_product():
+    A = O.laplacian3d(7, 6, 5, 4)
+    p = fcg_params("BCMG", aggrsweeps=0, max_coarse_size=10)
+    P = O.Prec(A, p)
+    S = A.to_scipy()
+    for l in range(P.levels() - 1):
+        agg, pv = P.level_P(l)
+        Al = P.level_A(l).to_scipy().toarray()
+        Pm = sp.csr_matrix((pv, agg, np.arange(len(agg) + 1)), shape=(len(agg), P.level_A(l + 1).dims()[0])).toarray()
+        Ac = P.level_A(l + 1).to_scipy().toarray()
+        assert np.allclose(Pm.T @ Al @ Pm, Ac, rtol=1e-13, atol=1e-12)
+        assert np.allclose(Ac, Ac.T, rtol=0, atol=1e-13)
+    assert S.shape[0] == 120
+
+
+def test_hierarchy_shape_27p():
+    A = O.laplacian3d(27, 16, 16, 16)
+    P = O.Prec(A, fcg_params("BCMG"))
+    sizes = [P.level_A(l).dims()[0] for l in range(P.levels())]
+    assert sizes == [4096, 512]  # 3 pairwise sweeps -> 2x2x2 aggregates; 512 <= 639
+    st = P.stats()
+    assert st["max_coarse_size"] == 639 and abs(st["avg_cratio"] - 8.0) < 1e-12
+
+
+# --------------------------------------------------------------- whole solve
+@pytest.mark.parametrize("pts", [7, 27])
+@pytest.mark.parametrize("prec", ["BCMG", "AFSAI", "L1_JACOBI", "NONE"])
+def test_solve_matches_reference_fixture(golden, pts, prec):
+    """launchAndCompare.mk: ||x_ref - x|| / ||x_ref|| <= 1e-5, A x = 1, x0 = 0."""
+    A = O.laplacian3d(pts, 10, 10, 10)
+    p = fcg_params(prec)
+    P = O.Prec(A, p)
+    x, info = O.fcg(A, P, p, np.ones(1000))
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_1x1x1_sol.txt"))
+    assert info["retv"] == 0
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+@pytest.mark.parametrize("pts", [7, 27])
+def test_solve_matches_block_fixture(golden, pts):
+    A = O.laplacian3d(pts, 10, 10, 10, 2, 1, 1)
+    p = fcg_params("BCMG")
+    x, info = O.fcg(A, O.Prec(A, p), p, np.ones(2000))
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_2x1x1_sol.txt"))
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["BCMG", "AFSAI", "L1_JACOBI"])
+def test_solve_bcsstk09_fixture(golden, prec):
+    A = O.read_mtx(os.path.join(golden, "bcsstk09.mtx"))
+    n = A.dims()[0]
+    p = fcg_params(prec, itnlim=5000)
+    x, info = O.fcg(A, O.Prec(A, p), p, np.ones(n))
+    ref = np.loadtxt(os.path.join(golden, "bcsstk09_sol.txt"))
+    assert info["retv"] == 0
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+def test_fcg_reaches_1e8_and_counts_like_reference():
+    A = O.laplacian3d(7, 20, 20, 20)
+    n = 8000
+    xs = np.random.default_rng(0).uniform(-1, 1, n)
+    b = O.spmv(A, xs)
+    p = fcg_params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1)
+    x, info = O.fcg(A, O.Prec(A, p), p, b)
+    r = b - O.spmv(A, x)
+    assert np.linalg.norm(r) <= 1e-8 * info["del0"] * (1 + 1e-9)
+    assert abs(info["exit_res"] - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(r) + 1e-12
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+    # itnlim stops with retv -1 and niter = itnlim + 1
+    p2 = fcg_params("BCMG", rtol=1e-30, itnlim=3)
+    _, info2 = O.fcg(A, O.Prec(A, p2), p2, b)
+    assert info2["retv"] == -1 and info2["niter"] == 4
+
+
+def test_fp32_hierarchy_oracle_converges():
+    A = O.laplacian3d(7, 20, 20, 20)
+    b = O.spmv(A, np.random.default_rng(1).uniform(-1, 1, 8000))
+    p64 = fcg_params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1)
+    p32 = fcg_params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, precision=1)
+    _, i64 = O.fcg(A, O.Prec(A, p64), p64, b)
+    _, i32 = O.fcg(A, O.Prec(A, p32), p32, b)
+    assert i32["retv"] == 0 and abs(i32["niter"] - i64["niter"]) <= 1
+
+
+# -------------------------------------------------------------------- AFSAI
+def test_afsai_factor_invariants():
+    A = O.laplacian3d(7, 8, 7, 6)
+    P = O.Prec(A, fcg_params("AFSAI"))
+    G = P.afsai_G().to_scipy()
+    S = A.to_scipy()
+    assert sp.triu(G, 1).nnz == 0
+    assert np.diff(G.indptr).max() <= 31
+    d = 1 / np.sqrt(S.diagonal())
+    As = sp.diags(d) @ S @ sp.diags(d)
+    dd = (G @ As @ G.T).diagonal()
+    assert np.allclose(dd, 1.0, rtol=1e-10)
+    # first step on the tied 7-point stencil: the entry with the smallest column wins
+    assert G.indices[G.indptr[S.shape[0] - 1]] < S.shape[0] - 1
