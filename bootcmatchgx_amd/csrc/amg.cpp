@@ -32,7 +32,8 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     auto it = c->tuned_lanes.find(sig);
     if (it != c->tuned_lanes.end()) return it->second;
     int base = default_lanes(A.nnz, A.n);
-    int cands[3] = {std::max(1, base / 2), base, std::min(64, base * 2)};
+    // vector kernels with L lanes per row, and CSR-stream kernels (negative: lanes per row reduction)
+    int cands[6] = {std::max(1, base / 2), base, std::min(64, base * 2), -1, -2, -4};
     hipEvent_t e0, e1;
     BCMGX_HIP(hipEventCreate(&e0));
     BCMGX_HIP(hipEventCreate(&e1));

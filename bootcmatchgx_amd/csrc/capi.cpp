@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -12,8 +13,16 @@
 
 using namespace bcmgx;
 
+struct bcmgx_matrix_s;
+struct bcmgx_prec_s;
+// A context owns the device memory of every handle created on it.  Destroying
+// the context releases that memory and orphans the handles (ctx = nullptr);
+// their own destroy calls then only free the host structures, so any
+// destruction order is safe.
 struct bcmgx_context_s {
     Context* c = nullptr;
+    std::set<bcmgx_matrix_s*> mats;
+    std::set<bcmgx_prec_s*> precs;
 };
 struct bcmgx_matrix_s {
     bcmgx_context_s* ctx = nullptr;
@@ -126,6 +135,15 @@ int bcmgx_context_destroy(bcmgx_context ctx) {
     if (!ctx) return BCMGX_OK;
     API_BEGIN
     (void)hipSetDevice(ctx->c->device);
+    ctx->c->sync();
+    for (auto* p : ctx->precs) {
+        p->P.reset();
+        p->ctx = nullptr;
+    }
+    for (auto* m : ctx->mats) {
+        m->A = DCsr();
+        m->ctx = nullptr;
+    }
     delete ctx->c;
     delete ctx;
     return BCMGX_OK;
@@ -193,6 +211,7 @@ int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, c
         delete h;
         throw;
     }
+    ctx->mats.insert(h);
     *out = h;
     return BCMGX_OK;
     API_END(ctx)
@@ -201,8 +220,11 @@ int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, c
 int bcmgx_csr_destroy(bcmgx_matrix A) {
     if (!A) return BCMGX_OK;
     API_BEGIN
-    (void)hipSetDevice(A->ctx->c->device);
-    A->ctx->c->sync();
+    if (A->ctx) {
+        (void)hipSetDevice(A->ctx->c->device);
+        A->ctx->c->sync();
+        A->ctx->mats.erase(A);
+    }
     delete A;
     return BCMGX_OK;
     API_END(nullptr)
@@ -218,6 +240,7 @@ int bcmgx_csr_info(bcmgx_matrix A, int* n, int* m, int* nnz) {
 
 int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val) {
     if (!A) return BCMGX_ERR_ARG;
+    if (!A->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Context* c = A->ctx->c;
     if (row) c->d2h(row, A->A.row.get(), (size_t)A->A.n + 1);
@@ -243,6 +266,7 @@ int bcmgx_prec_setup(bcmgx_context ctx, bcmgx_matrix A, const bcmgx_params* p, b
         delete h;
         throw;
     }
+    ctx->precs.insert(h);
     *out = h;
     return BCMGX_OK;
     API_END(ctx)
@@ -250,6 +274,7 @@ int bcmgx_prec_setup(bcmgx_context ctx, bcmgx_matrix A, const bcmgx_params* p, b
 
 int bcmgx_prec_apply(bcmgx_prec prec, const double* r, double* u, int nrhs, int ld, int mem) {
     if (!prec || !r || !u || nrhs < 1 || nrhs > 64) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Context* c = prec->ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
@@ -269,8 +294,11 @@ int bcmgx_prec_apply(bcmgx_prec prec, const double* r, double* u, int nrhs, int 
 int bcmgx_prec_finalize(bcmgx_prec prec) {
     if (!prec) return BCMGX_OK;
     API_BEGIN
-    (void)hipSetDevice(prec->ctx->c->device);
-    prec->ctx->c->sync();
+    if (prec->ctx) {
+        (void)hipSetDevice(prec->ctx->c->device);
+        prec->ctx->c->sync();
+        prec->ctx->precs.erase(prec);
+    }
     delete prec;
     return BCMGX_OK;
     API_END(nullptr)
@@ -278,6 +306,7 @@ int bcmgx_prec_finalize(bcmgx_prec prec) {
 
 int bcmgx_prec_get_info(bcmgx_prec prec, bcmgx_prec_info* info) {
     if (!prec || !info) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     std::memset(info, 0, sizeof(*info));
     Prec* P = prec->P.get();
     info->ptype = P->ptype;
@@ -293,6 +322,7 @@ int bcmgx_prec_get_info(bcmgx_prec prec, bcmgx_prec_info* info) {
 
 int bcmgx_prec_level_size(bcmgx_prec prec, int level, int* n, int* nnz) {
     if (!prec || level < 0) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     Prec* P = prec->P.get();
     if (P->ptype != BCMGX_PREC_BCMG || level >= P->H.levels()) return BCMGX_ERR_ARG;
     const DCsr& A = P->H.L[level]->mat();
@@ -303,6 +333,7 @@ int bcmgx_prec_level_size(bcmgx_prec prec, int level, int* n, int* nnz) {
 
 int bcmgx_prec_level_matrix(bcmgx_prec prec, int level, int* row, int* col, double* val) {
     if (!prec || level < 0) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Prec* P = prec->P.get();
     BCMGX_CHECK(P->ptype == BCMGX_PREC_BCMG && level < P->H.levels(), BCMGX_ERR_ARG, "no such level");
@@ -317,6 +348,7 @@ int bcmgx_prec_level_matrix(bcmgx_prec prec, int level, int* row, int* col, doub
 
 int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* pval) {
     if (!prec || level < 0) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Prec* P = prec->P.get();
     BCMGX_CHECK(P->ptype == BCMGX_PREC_BCMG && level < P->H.levels() - 1, BCMGX_ERR_ARG, "no prolongator at this level");
@@ -331,6 +363,7 @@ int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* p
 
 int bcmgx_prec_afsai_factor(bcmgx_prec prec, int* n, int* nnz, int* row, int* col, double* val) {
     if (!prec) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Prec* P = prec->P.get();
     BCMGX_CHECK(P->ptype == BCMGX_PREC_AFSAI, BCMGX_ERR_STATE, "not an AFSAI preconditioner");
@@ -347,6 +380,7 @@ int bcmgx_prec_afsai_factor(bcmgx_prec prec, int* n, int* nnz, int* row, int* co
 int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_params* p, const double* b, double* x, int nrhs,
                 int ld, int mem, bcmgx_solver_out* out) {
     if (!ctx || !A || !p || !b || !x || nrhs < 1 || nrhs > 64) return BCMGX_ERR_ARG;
+    if (A->ctx != ctx || (prec && prec->ctx != ctx)) return BCMGX_ERR_STATE;
     API_BEGIN
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
@@ -388,6 +422,7 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
 
 int bcmgx_solve_columns(bcmgx_prec prec, int nrhs, int* niter, double* exit_res, double* rel_res) {
     if (!prec) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
     Prec* P = prec->P.get();
     if ((int)P->last_niter.size() < nrhs) return BCMGX_ERR_STATE;
     for (int j = 0; j < nrhs; j++) {
@@ -481,6 +516,7 @@ int bcmgx_spgemm(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix B, bcmgx_matrix
         delete h;
         throw;
     }
+    ctx->mats.insert(h);
     *C = h;
     return BCMGX_OK;
     API_END(ctx)
@@ -500,6 +536,7 @@ int bcmgx_transpose(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix* T) {
         delete h;
         throw;
     }
+    ctx->mats.insert(h);
     *T = h;
     return BCMGX_OK;
     API_END(ctx)

@@ -144,21 +144,24 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
     for (int j = Wrow[i] + lane; j < Wrow[i + 1]; j += L) {
         int cc = Wcol[j];
         if (cc >= i) break;
-        int b = Wrow[cc];
-        int p = lower_bound_dev(Wcol + b, Wrow[cc + 1] - b, i);
-        Wval[j] = Wval[b + p];
+        int b = Wrow[cc], len = Wrow[cc + 1] - b;
+        int p = lower_bound_dev(Wcol + b, len, i);
+        if (p < len && Wcol[b + p] == i) Wval[j] = Wval[b + p];  // mirrored entry present
     }
 }
 
 // Preference ranks.  Vertex y orders its neighbours u by (W(y,u), u) -- the
 // Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
 // rank[e] = position of entry e of row y in that order; inv[Wrow[y] + r] = u.
-__global__ void k_rank(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol, const double* __restrict__ Wval,
-                       int* __restrict__ rank, int* __restrict__ inv) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x;
+// G lanes per row, each ranking a strided subset of the row's entries.
+template <int G>
+__global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
+                                             const double* __restrict__ Wval, int* __restrict__ rank, int* __restrict__ inv) {
+    int y = (blockIdx.x * WG + (int)threadIdx.x) / G;
+    int lane = (int)threadIdx.x % G;
     if (y >= n) return;
     int s = Wrow[y], e = Wrow[y + 1];
-    for (int a = s; a < e; a++) {
+    for (int a = s + lane; a < e; a += G) {
         double wa = Wval[a];
         int ca = Wcol[a];
         int r = 0;
@@ -172,15 +175,18 @@ __global__ void k_rank(int n, const int* __restrict__ Wrow, const int* __restric
 }
 
 // tkey[e] for entry e = (u, y) of row u: 1 + rank of u among y's neighbours
-__global__ void k_tkey(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol, const int* __restrict__ rank,
-                       int* __restrict__ tkey) {
-    int u = blockIdx.x * blockDim.x + threadIdx.x;
+template <int G>
+__global__ __launch_bounds__(WG) void k_tkey(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
+                                             const int* __restrict__ rank, int* __restrict__ tkey) {
+    int u = (blockIdx.x * WG + (int)threadIdx.x) / G;
+    int lane = (int)threadIdx.x % G;
     if (u >= n) return;
-    for (int e = Wrow[u]; e < Wrow[u + 1]; e++) {
+    for (int e = Wrow[u] + lane; e < Wrow[u + 1]; e += G) {
         int y = Wcol[e];
-        int b = Wrow[y];
-        int p = lower_bound_dev(Wcol + b, Wrow[y + 1] - b, u);
-        tkey[e] = rank[b + p] + 1;
+        int b = Wrow[y], len = Wrow[y + 1] - b;
+        int p = lower_bound_dev(Wcol + b, len, u);
+        // no mirrored entry: key 0 never beats an offer, u cannot propose to y
+        tkey[e] = (p < len && Wcol[b + p] == u) ? rank[b + p] + 1 : 0;
     }
 }
 
@@ -193,15 +199,18 @@ __global__ void k_tkey(int n, const int* __restrict__ Wrow, const int* __restric
 // another offer raised S[best], and offers only rise, so the grid drains.
 // The fixed point is the unique stable (greedy) matching of the total order
 // (W, max id, min id), whatever the interleaving.
-__global__ void k_suitor(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol, const double* __restrict__ Wval,
-                         const int* __restrict__ tkey, const int* __restrict__ inv, int* S) {
-    int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
+template <int G>
+__global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
+                                               const double* __restrict__ Wval, const int* __restrict__ tkey,
+                                               const int* __restrict__ inv, int* S) {
+    int u = (blockIdx.x * WG + (int)threadIdx.x) / G;
+    int lane = (int)threadIdx.x % G;
+    if (u >= n) return;  // whole G-lane group
     int cur = u;
-    while (cur >= 0) {
+    while (cur >= 0) {  // uniform inside the group
         int best = -1, bkey = 0;
         double bw = 0.0;
-        for (int e = Wrow[cur]; e < Wrow[cur + 1]; e++) {
+        for (int e = Wrow[cur] + lane; e < Wrow[cur + 1]; e += G) {
             double wv = Wval[e];
             if (!(wv > 0)) continue;
             int y = Wcol[e];
@@ -215,8 +224,22 @@ __global__ void k_suitor(int n, const int* __restrict__ Wrow, const int* __restr
                 bkey = k;
             }
         }
+        // heaviest candidate of the group, ties to the higher id (find_Candidate's reduction)
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            double pw = __shfl_xor(bw, o, G);
+            int pb = __shfl_xor(best, o, G);
+            int pk = __shfl_xor(bkey, o, G);
+            if (pb >= 0 && (best < 0 || pw > bw || (pw == bw && pb > best))) {
+                best = pb;
+                bw = pw;
+                bkey = pk;
+            }
+        }
         if (best < 0) break;
-        int old = atomicMax(&S[best], bkey);
+        int old = 0;
+        if (lane == 0) old = atomicMax(&S[best], bkey);
+        old = __shfl(old, 0, G);
         if (old < bkey) cur = old > 0 ? inv[Wrow[best] + old - 1] : -1;
     }
 }
@@ -229,6 +252,11 @@ __global__ void k_suitor_decode(int n, const int* __restrict__ Wrow, const int* 
     M[y] = s > 0 ? inv[Wrow[y] + s - 1] : -1;
 }
 
+
+__global__ void k_count_missing_diag(int n, const int* __restrict__ dind, int* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && dind[i] < 0) atomicAdd(cnt, 1);
+}
 
 __global__ void k_make_wrow(int n, const int* __restrict__ row, int* __restrict__ Wrow) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -250,6 +278,13 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     size_t mm = (size_t)std::max<long long>(m, 1);
     DBuf<int> dind(c, n);
     diag_index(c, n, A.row.get(), A.col.get(), dind.get());
+    {  // makeAH requires a diagonal entry in every row (matching.cu:155)
+        DBuf<int> miss(c, 1);
+        fill<int>(c, miss.get(), 1, 0);
+        k_count_missing_diag<<<grid_for(n, WG), WG, 0, c->stream>>>(n, dind.get(), miss.get());
+        BCMGX_LAUNCHED();
+        BCMGX_CHECK(c->read1(miss.get()) == 0, BCMGX_ERR_ARG, "matching: every row needs a diagonal entry");
+    }
     DBuf<double> C(c, n);
     k_make_C<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.val.get(), dind.get(), w, C.get());
     BCMGX_LAUNCHED();
@@ -286,14 +321,20 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
 #undef BCMGX_SY
     BCMGX_LAUNCHED();
     DBuf<int> rank(c, mm), inv(c, mm), tkey(c, mm);
-    k_rank<<<grid_for(n, 64), 64, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(), inv.get());
-    BCMGX_LAUNCHED();
-    k_tkey<<<grid_for(n, 64), 64, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), tkey.get());
-    BCMGX_LAUNCHED();
     DBuf<int> S(c, n);
     fill<int>(c, S.get(), n, 0);
-    k_suitor<<<grid_for(n, 64), 64, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get());
-    BCMGX_LAUNCHED();
+#define BCMGX_MT(LL)                                                                                                   \
+    case LL:                                                                                                           \
+        k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(), inv.get()); \
+        BCMGX_LAUNCHED();                                                                                              \
+        k_tkey<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), tkey.get());     \
+        BCMGX_LAUNCHED();                                                                                              \
+        k_suitor<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(),   \
+                                                                 inv.get(), S.get());                                  \
+        BCMGX_LAUNCHED();                                                                                              \
+        break;
+    switch (L) { BCMGX_MT(4) BCMGX_MT(8) BCMGX_MT(16) BCMGX_MT(32) }
+#undef BCMGX_MT
     k_suitor_decode<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Wrow.get(), inv.get(), S.get(), M);
     BCMGX_LAUNCHED();
 }
@@ -488,15 +529,27 @@ __global__ void k_count_products(int n, const int* __restrict__ arow, const int*
     cnt[i] = s;
 }
 
-__global__ void k_bin_rows(int n, const int* __restrict__ arow, const long long* __restrict__ cnt, int* __restrict__ bins,
-                           int* __restrict__ bincount, int cap) {
+// bin rows by product count; one atomic per wave and bin (order inside a bin is irrelevant)
+__global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ arow, const long long* __restrict__ cnt,
+                                                 int* __restrict__ bins, int* __restrict__ bincount, int cap) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    long long p = cnt[i];
-    int alen = arow[i + 1] - arow[i];
-    int b = (p <= SMALL_P && alen <= SMALL_P) ? 0 : ((p <= MED_P && alen <= MED_P) ? 1 : 2);
-    int pos = atomicAdd(&bincount[b], 1);
-    bins[(long long)b * cap + pos] = i;
+    int lane = threadIdx.x & 63;
+    int b = -1;
+    if (i < n) {
+        long long p = cnt[i];
+        int alen = arow[i + 1] - arow[i];
+        b = (p <= SMALL_P && alen <= SMALL_P) ? 0 : ((p <= MED_P && alen <= MED_P) ? 1 : 2);
+    }
+    unsigned long long below = (1ull << lane) - 1ull;
+    for (int bb = 0; bb < 3; bb++) {
+        unsigned long long m = __ballot(b == bb);
+        if (!m) continue;
+        int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&bincount[bb], __popcll(m));
+        base = __shfl(base, leader, 64);
+        if (b == bb) bins[(long long)bb * cap + base + __popcll(m & below)] = i;
+    }
 }
 
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {

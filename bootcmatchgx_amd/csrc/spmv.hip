@@ -54,21 +54,93 @@ __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, const int* __restric
     int blk = xcd_remap(blockIdx.x, gridDim.x);
     int r = blk * RPB + (int)threadIdx.x / KP;
     int c = (int)threadIdx.x % KP;
-    if (r >= n || c >= k) return;
+    if (r >= n) return;  // the whole KP-lane group leaves together
     int s = row[r], e = row[r + 1];
     V t = 0;
-#pragma unroll 4
-    for (int j = s; j < e; j++) {
-        V a = (V)val[j];
-        long long cc = col[j];
-        if (OP == SpOp::AXPBY) t += (alpha * a) * x[cc * k + c];
-        else t += a * x[cc * k + c];
+    // the group loads KP (col, val) pairs at once, then broadcasts them, so
+    // the x gathers of a chunk are independent loads
+    for (int base = s; base < e; base += KP) {
+        int j = base + c;
+        int cl = 0;
+        V al = 0;
+        if (j < e) {
+            cl = ldg_nt(col + j);
+            al = (V)ldg_nt(val + j);
+        }
+        int cnt = e - base < KP ? e - base : KP;
+#pragma unroll 8
+        for (int q = 0; q < cnt; q++) {
+            long long cc = __shfl(cl, q, KP);
+            V a = __shfl(al, q, KP);
+            if (c < k) {
+                if (OP == SpOp::AXPBY) t += (alpha * a) * x[cc * k + c];
+                else t += a * x[cc * k + c];
+            }
+        }
     }
+    if (c >= k) return;
     long long o = (long long)r * k + c;
     if (OP == SpOp::AX) y[o] = t;
     else if (OP == SpOp::RES) y[o] = b[o] - t;
     else if (OP == SpOp::AXPBY) y[o] = t + beta * y[o];
     else y[o] = ((-t + b[o]) / D[r]) + x[o];
+}
+
+// CSR-stream SpMV for one vector: a workgroup owns 256/TPR consecutive rows;
+// the products of their (contiguous) nonzeros are formed with fully
+// coalesced streaming loads into LDS, then TPR lanes reduce each row.  Blocks
+// whose nonzeros exceed the LDS capacity reduce straight from global memory.
+constexpr int STREAM_CAP = 4096;
+template <typename T, typename V, int TPR, SpOp OP>
+__global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                    const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
+                                                    const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
+                                                    const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    constexpr int RB = WG / TPR;
+    __shared__ V prod[STREAM_CAP];
+    __shared__ int rp[RB + 1];
+    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int r0 = blk * RB;
+    int tid = threadIdx.x;
+    for (int t = tid; t <= RB; t += WG) {
+        int rr = r0 + t;
+        rp[t] = row[rr < n ? rr : n];
+    }
+    __syncthreads();
+    int base = rp[0], cnt = rp[RB] - base;
+    bool staged = cnt <= STREAM_CAP;
+    if (staged) {
+        for (int t = tid; t < cnt; t += WG) {
+            int j = base + t;
+            V a = (V)ldg_nt(val + j);
+            if (OP == SpOp::AXPBY) a = alpha * a;
+            prod[t] = a * x[ldg_nt(col + j)];
+        }
+    }
+    __syncthreads();
+    int lr = tid / TPR, sub = tid % TPR;
+    int r = r0 + lr;
+    if (r >= n) return;  // whole TPR-lane group
+    int s = rp[lr], e = rp[lr + 1];
+    V acc = 0;
+    if (staged) {
+        for (int q = s + sub; q < e; q += TPR) acc += prod[q - base];
+    } else {
+        for (int q = s + sub; q < e; q += TPR) {
+            V a = (V)val[q];
+            if (OP == SpOp::AXPBY) a = alpha * a;
+            acc += a * x[col[q]];
+        }
+    }
+#pragma unroll
+    for (int o = TPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, TPR);
+    if (sub == 0) {
+        if (OP == SpOp::AX) y[r] = acc;
+        else if (OP == SpOp::RES) y[r] = b[r] - acc;
+        else if (OP == SpOp::AXPBY) y[r] = acc + beta * y[r];
+        else y[r] = ((-acc + b[r]) / D[r]) + x[r];
+    }
 }
 
 int default_lanes(long long nnz, int n) {
@@ -91,6 +163,20 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
     const V* b = (const V*)a.b;
     const V* D = (const V*)a.D;
     V al = (V)a.alpha, be = (V)a.beta;
+    if (L < 0) {  // CSR-stream variant, TPR = -L lanes per row
+#define BCMGX_S(TT)                                                                                                 \
+    case TT:                                                                                                        \
+        k_spmv_stream<T, V, TT, OP><<<grid_for(a.n, WG / TT), WG, 0, c->stream>>>(a.n, a.row, a.col, val, x, y, b, D, \
+                                                                                 al, be, a.stop);                   \
+        break;
+        switch (-L) {
+            BCMGX_S(1) BCMGX_S(2) BCMGX_S(4) BCMGX_S(8) BCMGX_S(16)
+        default: throw Error(BCMGX_ERR_ARG, "stream lanes per row must be a power of two <= 16");
+        }
+#undef BCMGX_S
+        BCMGX_LAUNCHED();
+        return;
+    }
 #define BCMGX_L(LL)                                                                                            \
     case LL:                                                                                                   \
         k_spmv1<T, V, LL, OP><<<grid_for(a.n, WG / LL), WG, 0, c->stream>>>(a.n, a.row, a.col, val, x, y, b, D, \
@@ -131,7 +217,7 @@ template <typename T, typename V>
 void spmv(Context* c, SpOp op, const SpmvArgs& a) {
     if (a.n <= 0) return;
     if (a.k == 1) {
-        int L = a.lanes > 0 ? a.lanes : default_lanes(a.nnz, a.n);
+        int L = a.lanes != 0 ? a.lanes : default_lanes(a.nnz, a.n);  // < 0: CSR-stream kernel
         switch (op) {
         case SpOp::AX: launch1<T, V, SpOp::AX>(c, a, L); break;
         case SpOp::RES: launch1<T, V, SpOp::RES>(c, a, L); break;

@@ -388,7 +388,7 @@ void matching_weights(const Csr& A, const double* w, std::vector<int>& Wrow, std
             const int* b = Wcol.data() + Wrow[c];
             int len = Wrow[c + 1] - Wrow[c];
             int p = (int)(std::lower_bound(b, b + len, i) - b);
-            Wval[j] = Wval[Wrow[c] + p];
+            if (p < len && b[p] == i) Wval[j] = Wval[Wrow[c] + p];  // the reference assumes the mirror exists
         }
 }
 

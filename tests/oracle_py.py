@@ -192,7 +192,9 @@ class Prec:
         return lib().or_prec_levels(self.h)
 
     def level_A(self, l):
-        return Csr(lib().or_prec_level_A(self.h, l), owned=False)
+        c = Csr(lib().or_prec_level_A(self.h, l), owned=False)
+        c.parent = self  # the matrix lives inside this preconditioner
+        return c
 
     def level_P(self, l):
         n, _, _ = self.level_A(l).dims()
@@ -207,7 +209,9 @@ class Prec:
         return dict(op_cmplx=a.value, op_wcmplx=b.value, avg_cratio=c.value, max_coarse_size=d.value)
 
     def afsai_G(self):
-        return Csr(lib().or_prec_afsai_G(self.h), owned=False)
+        c = Csr(lib().or_prec_afsai_G(self.h), owned=False)
+        c.parent = self
+        return c
 
     def apply(self, r, u=None):
         n, _, _ = self.A.dims()
