@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 one-at-a-time comparison")
+    ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
+                    help="only time REPS launches of the FP64 SpMV of the fine matrix (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -93,6 +95,11 @@ def main():
     nnz = len(col)
     A = bx.CSR(ctx, n, n, row, col, val)
     del row, col, val
+    if args.spmv_only:
+        ms = A.time_spmv(precision=0, k=1, reps=args.spmv_only)
+        gbs = spmv_fp64_bytes(n, nnz) / (ms * 1e-3) / 1e9
+        print(json.dumps({"kernel": "spmv fp64", "n": n, "nnz": nnz, "avg_launch_ms": ms, "gbs": gbs}))
+        return
     Bh = np.asfortranarray(np.random.default_rng(3).normal(size=(n, k)))
     B = bx.DeviceArray(ctx, (n, k))
     B.upload(Bh.ravel(order="F"))

@@ -199,48 +199,64 @@ __global__ __launch_bounds__(WG) void k_tkey(int n, const int* __restrict__ Wrow
 // another offer raised S[best], and offers only rise, so the grid drains.
 // The fixed point is the unique stable (greedy) matching of the total order
 // (W, max id, min id), whatever the interleaving.
+// One round of the lock-free Suitor.  Each G-lane group takes a vertex from
+// the round's work list (all vertices in round 0) and follows its proposal
+// chain for at most `maxsteps` proposals; a vertex still unsettled then goes
+// to the next round's list.  Bounding the steps keeps one long displacement
+// chain from serialising a whole launch: every round works on all pending
+// vertices in parallel with fresh offers.
 template <int G>
 __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
                                                const double* __restrict__ Wval, const int* __restrict__ tkey,
-                                               const int* __restrict__ inv, int* S) {
-    int u = (blockIdx.x * WG + (int)threadIdx.x) / G;
+                                               const int* __restrict__ inv, int* S, const int* __restrict__ in_list,
+                                               const int* __restrict__ in_count, int* __restrict__ out_list,
+                                               int* __restrict__ out_count, int maxsteps) {
     int lane = (int)threadIdx.x % G;
-    if (u >= n) return;  // whole G-lane group
-    int cur = u;
-    while (cur >= 0) {  // uniform inside the group
-        int best = -1, bkey = 0;
-        double bw = 0.0;
-        for (int e = Wrow[cur] + lane; e < Wrow[cur + 1]; e += G) {
-            double wv = Wval[e];
-            if (!(wv > 0)) continue;
-            int y = Wcol[e];
-            if (y < 0 || y >= n) continue;
-            int k = tkey[e];
-            int sy = __hip_atomic_load(&S[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (sy >= k) continue;
-            if (best < 0 || wv > bw || (wv == bw && y > best)) {
-                best = y;
-                bw = wv;
-                bkey = k;
+    int ngroups = gridDim.x * (WG / G);
+    int total = in_list ? *in_count : n;
+    for (int it = (blockIdx.x * WG + (int)threadIdx.x) / G; it < total; it += ngroups) {
+        int cur = in_list ? in_list[it] : it;
+        int steps = 0;
+        while (cur >= 0) {  // uniform inside the group
+            if (steps == maxsteps) {
+                if (lane == 0) out_list[atomicAdd(out_count, 1)] = cur;
+                break;
             }
-        }
-        // heaviest candidate of the group, ties to the higher id (find_Candidate's reduction)
+            steps++;
+            int best = -1, bkey = 0;
+            double bw = 0.0;
+            for (int e = Wrow[cur] + lane; e < Wrow[cur + 1]; e += G) {
+                double wv = Wval[e];
+                if (!(wv > 0)) continue;
+                int y = Wcol[e];
+                if (y < 0 || y >= n) continue;
+                int k = tkey[e];
+                int sy = __hip_atomic_load(&S[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (sy >= k) continue;
+                if (best < 0 || wv > bw || (wv == bw && y > best)) {
+                    best = y;
+                    bw = wv;
+                    bkey = k;
+                }
+            }
+            // heaviest candidate of the group, ties to the higher id (find_Candidate's reduction)
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            double pw = __shfl_xor(bw, o, G);
-            int pb = __shfl_xor(best, o, G);
-            int pk = __shfl_xor(bkey, o, G);
-            if (pb >= 0 && (best < 0 || pw > bw || (pw == bw && pb > best))) {
-                best = pb;
-                bw = pw;
-                bkey = pk;
+            for (int o = G / 2; o > 0; o >>= 1) {
+                double pw = __shfl_xor(bw, o, G);
+                int pb = __shfl_xor(best, o, G);
+                int pk = __shfl_xor(bkey, o, G);
+                if (pb >= 0 && (best < 0 || pw > bw || (pw == bw && pb > best))) {
+                    best = pb;
+                    bw = pw;
+                    bkey = pk;
+                }
             }
+            if (best < 0) break;
+            int old = 0;
+            if (lane == 0) old = atomicMax(&S[best], bkey);
+            old = __shfl(old, 0, G);
+            if (old < bkey) cur = old > 0 ? inv[Wrow[best] + old - 1] : -1;
         }
-        if (best < 0) break;
-        int old = 0;
-        if (lane == 0) old = atomicMax(&S[best], bkey);
-        old = __shfl(old, 0, G);
-        if (old < bkey) cur = old > 0 ? inv[Wrow[best] + old - 1] : -1;
     }
 }
 
@@ -261,6 +277,32 @@ __global__ void k_count_missing_diag(int n, const int* __restrict__ dind, int* _
 __global__ void k_make_wrow(int n, const int* __restrict__ row, int* __restrict__ Wrow) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= n) Wrow[i] = row[i] - i;
+}
+
+// Rounds of k_suitor until no vertex is pending.  The list length is read
+// back every few rounds (an empty round costs one tiny launch).
+template <int G>
+static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, const double* Wval, const int* tkey,
+                          const int* inv, int* S) {
+    const int maxsteps = 8;
+    DBuf<int> la(c, n), lb(c, n), cnt(c, 2);
+    fill<int>(c, cnt.get(), 2, 0);
+    unsigned grid = (unsigned)std::min<long long>(grid_for(n, WG / G), (long long)c->num_cus * 16);
+    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, nullptr, nullptr, la.get(), cnt.get(), maxsteps);
+    BCMGX_LAUNCHED();
+    int* lists[2] = {la.get(), lb.get()};
+    int cur = 0;
+    for (int round = 1;; round++) {
+        // lists[cur] / cnt[cur] hold the pending vertices; clear the other counter and run a round
+        fill<int>(c, cnt.get() + (cur ^ 1), 1, 0);
+        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, lists[cur], cnt.get() + cur,
+                                                 lists[cur ^ 1], cnt.get() + (cur ^ 1), maxsteps);
+        BCMGX_LAUNCHED();
+        cur ^= 1;
+        if (round % 8 == 0 && c->read1(cnt.get() + cur) == 0) break;
+        // every round settles at least one proposal; offers are bounded by the edge count
+        BCMGX_CHECK(round < 4 * n + 1024, BCMGX_ERR_NUMERIC, "matching did not settle");
+    }
 }
 
 static int lanes_for(long long nnz, int n) {
@@ -329,9 +371,7 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
         BCMGX_LAUNCHED();                                                                                              \
         k_tkey<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), tkey.get());     \
         BCMGX_LAUNCHED();                                                                                              \
-        k_suitor<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(),   \
-                                                                 inv.get(), S.get());                                  \
-        BCMGX_LAUNCHED();                                                                                              \
+        suitor_rounds<LL>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get());                 \
         break;
     switch (L) { BCMGX_MT(4) BCMGX_MT(8) BCMGX_MT(16) BCMGX_MT(32) }
 #undef BCMGX_MT

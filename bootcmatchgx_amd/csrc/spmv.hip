@@ -51,34 +51,38 @@ __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, const int* __restric
                                               const int* __restrict__ stop) {
     if (stop && *stop) return;
     constexpr int RPB = WG / KP;
+    constexpr int U = 8;  // independent (col, val) -> x gathers in flight per lane
     int blk = xcd_remap(blockIdx.x, gridDim.x);
     int r = blk * RPB + (int)threadIdx.x / KP;
     int c = (int)threadIdx.x % KP;
-    if (r >= n) return;  // the whole KP-lane group leaves together
+    if (r >= n || c >= k) return;
     int s = row[r], e = row[r + 1];
+    const V* xc = x + c;
     V t = 0;
-    // the group loads KP (col, val) pairs at once, then broadcasts them, so
-    // the x gathers of a chunk are independent loads
-    for (int base = s; base < e; base += KP) {
-        int j = base + c;
-        int cl = 0;
-        V al = 0;
-        if (j < e) {
-            cl = ldg_nt(col + j);
-            al = (V)ldg_nt(val + j);
+    int j = s;
+    for (; j + U <= e; j += U) {
+        // every lane of the group reads the same (col, val): one broadcast access per wave
+        long long cc[U];
+        V a[U], xv[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            cc[q] = col[j + q];
+            a[q] = (V)val[j + q];
         }
-        int cnt = e - base < KP ? e - base : KP;
-#pragma unroll 8
-        for (int q = 0; q < cnt; q++) {
-            long long cc = __shfl(cl, q, KP);
-            V a = __shfl(al, q, KP);
-            if (c < k) {
-                if (OP == SpOp::AXPBY) t += (alpha * a) * x[cc * k + c];
-                else t += a * x[cc * k + c];
-            }
+#pragma unroll
+        for (int q = 0; q < U; q++) xv[q] = xc[cc[q] * k];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            if (OP == SpOp::AXPBY) t += (alpha * a[q]) * xv[q];
+            else t += a[q] * xv[q];
         }
     }
-    if (c >= k) return;
+    for (; j < e; j++) {
+        V a = (V)val[j];
+        long long cc = col[j];
+        if (OP == SpOp::AXPBY) t += (alpha * a) * xc[cc * k];
+        else t += a * xc[cc * k];
+    }
     long long o = (long long)r * k + c;
     if (OP == SpOp::AX) y[o] = t;
     else if (OP == SpOp::RES) y[o] = b[o] - t;
