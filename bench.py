@@ -98,7 +98,10 @@ def main():
     if args.spmv_only:
         ms = A.time_spmv(precision=0, k=1, reps=args.spmv_only)
         gbs = spmv_fp64_bytes(n, nnz) / (ms * 1e-3) / 1e9
-        print(json.dumps({"kernel": "spmv fp64", "n": n, "nnz": nnz, "avg_launch_ms": ms, "gbs": gbs}))
+        copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)  # k_copy16 over 1 GiB: counter calibration
+        print(json.dumps({"kernel": "spmv fp64", "n": n, "nnz": nnz, "avg_launch_ms": ms, "gbs": gbs,
+                          "algorithmic_bytes": spmv_fp64_bytes(n, nnz), "copy_bytes_per_launch": 2 * (1 << 30),
+                          "copy_gbs": copy_gbs}))
         return
     Bh = np.asfortranarray(np.random.default_rng(3).normal(size=(n, k)))
     B = bx.DeviceArray(ctx, (n, k))

@@ -33,7 +33,7 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     if (it != c->tuned_lanes.end()) return it->second;
     int base = default_lanes(A.nnz, A.n);
     // vector kernels with L lanes per row, and CSR-stream kernels (negative: lanes per row reduction)
-    int cands[6] = {std::max(1, base / 2), base, std::min(64, base * 2), -1, -2, -4};
+    int cands[6] = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8};
     hipEvent_t e0, e1;
     BCMGX_HIP(hipEventCreate(&e0));
     BCMGX_HIP(hipEventCreate(&e1));
@@ -71,6 +71,44 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
             BCMGX_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (ms < bestms) { bestms = ms; best = L; }
         }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tuned_lanes[sig] = best;
+    return best;
+}
+
+template <typename V>
+static float time_spmm(Context* c, const DCsr& A, const void* val, int k, int chunk, hipEvent_t e0, hipEvent_t e1) {
+    DBuf<V> x(c, (size_t)A.m * k), y(c, (size_t)A.n * k);
+    fill<V>(c, x.get(), (long long)A.m * k, (V)1);
+    SpmvArgs a;
+    a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = val; a.nnz = A.nnz;
+    a.x = x.get(); a.y = y.get(); a.k = k; a.kchunk = chunk;
+    spmv<V, V>(c, SpOp::AX, a);
+    BCMGX_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 3; r++) spmv<V, V>(c, SpOp::AX, a);
+    BCMGX_HIP(hipEventRecord(e1, c->stream));
+    BCMGX_HIP(hipEventSynchronize(e1));
+    float ms;
+    BCMGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+}
+
+int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
+    if (k <= 4 || A.n < 65536) return k;
+    std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d") + ":k" + std::to_string(k);
+    auto it = c->tuned_lanes.find(sig);
+    if (it != c->tuned_lanes.end()) return it->second;
+    hipEvent_t e0, e1;
+    BCMGX_HIP(hipEventCreate(&e0));
+    BCMGX_HIP(hipEventCreate(&e1));
+    int best = k;
+    float bestms = 1e30f;
+    for (int ch = k; ch >= 4; ch = (ch + 1) / 2) {
+        float ms = fp32 ? time_spmm<float>(c, A, A.valf.get(), k, ch, e0, e1) : time_spmm<double>(c, A, A.val.get(), k, ch, e0, e1);
+        if (ms < bestms) { bestms = ms; best = ch; }
+        if (ch == 4) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -196,6 +234,12 @@ static void make_fp32_copies(Context* c, Hierarchy& H) {
 Prec::~Prec() {}
 
 void Prec::ensure_work(int k) {
+    if (ptype == BCMGX_PREC_BCMG && k > 1 && k != tuned_k) {
+        // SpMM column chunk for this batch width (tuned outside any graph capture)
+        bool f32 = p.precision == BCMGX_PRECISION_FP32;
+        for (auto& lp : H.L) lp->chunkA = p.autotune ? tune_chunk(ctx, lp->mat(), f32, k) : k;
+        tuned_k = k;
+    }
     if (k <= kcap) return;
     kcap = k;
     if (ptype == BCMGX_PREC_BCMG) {
@@ -232,7 +276,7 @@ static const V* mat_val(const DCsr& A) {
 // guess (ignored when xzero).  Returns the buffer holding the result.
 template <typename V>
 static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const V* F, V* cur, V* oth, int s, bool xzero,
-                        int k, const int* stop) {
+                        int k, const int* stop, int chunk = 0) {
     int done = 0;
     if (s <= 0) {
         if (xzero) fill<V>(c, cur, (long long)A.n * k, (V)0);
@@ -245,7 +289,7 @@ static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const 
     for (; done < s; done++) {
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
-        a.x = cur; a.y = oth; a.b = F; a.D = M; a.k = k; a.lanes = lanes; a.stop = stop;
+        a.x = cur; a.y = oth; a.b = F; a.D = M; a.k = k; a.lanes = lanes; a.stop = stop; a.kchunk = chunk;
         spmv<V, V>(c, SpOp::JACOBI, a);
         std::swap(cur, oth);
     }
@@ -300,17 +344,19 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin) {
             convert_stop<double, float>(c, res, cur, nk, R.stop);
             return cur;
         } else {
-            return jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.relaxnumber_coarse, xzero, k, R.stop);
+            return jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.relaxnumber_coarse, xzero, k, R.stop,
+                                    lv.chunkA);
         }
     }
     int coeff = p.cycle_type == BCMGX_CYCLE_VARIABLE_V ? (1 << l) : 1;
-    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff, xzero, k, R.stop);
+    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff, xzero, k, R.stop,
+                           lv.chunkA);
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
     // Res = F - A x ; F_{l+1} = R Res
     {
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
-        a.x = cur; a.y = w.Res.get(); a.b = F; a.k = k; a.lanes = lv.lanesA; a.stop = R.stop;
+        a.x = cur; a.y = w.Res.get(); a.b = F; a.k = k; a.lanes = lv.lanesA; a.stop = R.stop; a.kchunk = lv.chunkA;
         spmv<V, V>(c, SpOp::RES, a);
     }
     LevelWork<V>& wc = (*R.W)[l + 1];
@@ -332,7 +378,8 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin) {
     }
     if (xc) prolong_add<V>(c, A.n, k, lv.agg.get(), lvl_pval<V>(lv), xc, cur, R.stop);
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
-    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.postrelax_sweeps * coeff, false, k, R.stop);
+    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.postrelax_sweeps * coeff, false, k, R.stop,
+                           lv.chunkA);
     return cur;
 }
 
@@ -444,6 +491,7 @@ struct SolveWork {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int lanesA = 0;
+    int chunkA = 0;
     ~SolveWork() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -459,7 +507,7 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     prec->apply(s.r.get(), P0, s.k, true, stop);  // P0 zero-filled in the reference (FCG.cu:106,119)
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
-    a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop;
+    a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
     spmv<double, double>(c, SpOp::AX, a);
     fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1);
     fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get());
@@ -493,6 +541,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         s.fb.st = s.st.get();
         s.fb.all_done = s.flags.get();
         s.lanesA = p.autotune ? tune_lanes(c, A, false) : default_lanes(A.nnz, A.n);
+        s.chunkA = p.autotune ? tune_chunk(c, A, false, k) : k;
     }
     SolveWork& s = *sp;
     s.fb.rtol = p.rtol;
@@ -506,7 +555,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
-        a.x = s.X.get(); a.y = s.r.get(); a.b = s.B.get(); a.k = k; a.lanes = s.lanesA;
+        a.x = s.X.get(); a.y = s.r.get(); a.b = s.B.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         spmv<double, double>(c, SpOp::RES, a);  // r = b - A x
     }
     fcg_norm(c, s.fb, s.r.get(), 0);           // delta0 = ||r||
@@ -514,7 +563,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
-        a.x = s.v.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA;
+        a.x = s.v.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         spmv<double, double>(c, SpOp::AX, a);  // w = A v
     }
     fcg_init_dots(c, s.fb, s.r.get(), s.v.get(), s.w.get());

@@ -19,6 +19,7 @@ struct Level {
     DBuf<float> pvalf;
     DCsr R;                 // restrictor P^T (rows sorted by fine index)
     int lanesA = 0, lanesR = 0;
+    int chunkA = 0;         // columns per SpMM launch for the current batch width
     const DCsr& mat() const { return Aref ? *Aref : A; }
 };
 
@@ -50,6 +51,7 @@ struct Prec {
     int lanesG = 0, lanesGT = 0;
     // work for the preconditioner application, sized for `kcap` columns
     int kcap = 0;
+    int tuned_k = 0;
     std::vector<LevelWork<double>> wd;
     std::vector<LevelWork<float>> wf;
     DBuf<double> cd_F, cd_X, cd_X2;  // FP64 coarsest-level buffers of the FP32 cycle
@@ -78,5 +80,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
 
 // autotune: fastest lanes per row for SpMV with A (k = 1)
 int tune_lanes(Context* c, const DCsr& A, bool fp32);
+// autotune: fastest column chunk for SpMM with A on k interleaved columns
+int tune_chunk(Context* c, const DCsr& A, bool fp32, int k);
 
 }  // namespace bcmgx

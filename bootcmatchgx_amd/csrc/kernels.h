@@ -41,7 +41,8 @@ struct SpmvArgs {
     const void* D = nullptr;    // V*  (JACOBI)
     double alpha = 1.0, beta = 0.0;
     int k = 1;                  // number of vector columns
-    int lanes = 0;              // lanes per row for k == 1 (0 = heuristic)
+    int lanes = 0;              // lanes per row for k == 1 (0 = heuristic, < 0 = CSR-stream kernel)
+    int kchunk = 0;             // k > 1: columns per launch (0 = all k)
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
 };
 // T = matrix value type, V = vector type: (double,double), (float,float), (float,double)

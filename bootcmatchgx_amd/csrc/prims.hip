@@ -277,8 +277,18 @@ void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int
 // sustained copy bandwidth (STREAM-copy style, 16 B per lane per access)
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(WG) void k_copy16(const dvec2* __restrict__ s, dvec2* __restrict__ d, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {  // four 16-byte loads in flight per lane
+        dvec2 v0 = __builtin_nontemporal_load(&s[i]);
+        dvec2 v1 = __builtin_nontemporal_load(&s[i + stride]);
+        dvec2 v2 = __builtin_nontemporal_load(&s[i + 2 * stride]);
+        dvec2 v3 = __builtin_nontemporal_load(&s[i + 3 * stride]);
+        __builtin_nontemporal_store(v0, &d[i]);
+        __builtin_nontemporal_store(v1, &d[i + stride]);
+        __builtin_nontemporal_store(v2, &d[i + 2 * stride]);
+        __builtin_nontemporal_store(v3, &d[i + 3 * stride]);
+    }
     for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(&s[i]), &d[i]);
 }
 double stream_copy_gbs(Context* c, size_t bytes, int reps) {

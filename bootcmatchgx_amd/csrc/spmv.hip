@@ -45,17 +45,18 @@ __global__ __launch_bounds__(WG) void k_spmv1(int n, const int* __restrict__ row
 }
 
 template <typename T, typename V, int KP, SpOp OP>
-__global__ __launch_bounds__(WG) void k_spmmk(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
-                                              const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
-                                              const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
-                                              const int* __restrict__ stop) {
+__global__ __launch_bounds__(WG) void k_spmmk(int n, int k, int c0, int kc, const int* __restrict__ row,
+                                              const int* __restrict__ col, const T* __restrict__ val,
+                                              const V* __restrict__ x, V* __restrict__ y, const V* __restrict__ b,
+                                              const V* __restrict__ D, V alpha, V beta, const int* __restrict__ stop) {
     if (stop && *stop) return;
     constexpr int RPB = WG / KP;
     constexpr int U = 8;  // independent (col, val) -> x gathers in flight per lane
     int blk = xcd_remap(blockIdx.x, gridDim.x);
     int r = blk * RPB + (int)threadIdx.x / KP;
-    int c = (int)threadIdx.x % KP;
-    if (r >= n || c >= k) return;
+    int cl = (int)threadIdx.x % KP;
+    if (r >= n || cl >= kc) return;
+    int c = c0 + cl;  // this lane's column among the k interleaved ones
     int s = row[r], e = row[r + 1];
     const V* xc = x + c;
     V t = 0;
@@ -90,11 +91,12 @@ __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, const int* __restric
     else y[o] = ((-t + b[o]) / D[r]) + x[o];
 }
 
-// CSR-stream SpMV for one vector: a workgroup owns 256/TPR consecutive rows;
-// the products of their (contiguous) nonzeros are formed with fully
-// coalesced streaming loads into LDS, then TPR lanes reduce each row.  Blocks
-// whose nonzeros exceed the LDS capacity reduce straight from global memory.
-constexpr int STREAM_CAP = 4096;
+// CSR-stream SpMV for one vector: a workgroup owns RB = 256/TPR consecutive
+// rows; the products of their (contiguous) nonzeros are formed with fully
+// coalesced streaming loads into LDS (four independent val/col/x loads per
+// thread in flight), then TPR lanes reduce each row from LDS.  The LDS tile
+// holds 32 nonzeros per row on average (so a block stays small enough for
+// full occupancy); blocks with more reduce straight from global memory.
 template <typename T, typename V, int TPR, SpOp OP>
 __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict__ row, const int* __restrict__ col,
                                                     const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
@@ -102,7 +104,9 @@ __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict
                                                     const int* __restrict__ stop) {
     if (stop && *stop) return;
     constexpr int RB = WG / TPR;
-    __shared__ V prod[STREAM_CAP];
+    constexpr int CAP = RB * 32 < 4096 ? RB * 32 : 4096;
+    constexpr int U = 4;
+    __shared__ V prod[CAP];
     __shared__ int rp[RB + 1];
     int blk = xcd_remap(blockIdx.x, gridDim.x);
     int r0 = blk * RB;
@@ -113,13 +117,28 @@ __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict
     }
     __syncthreads();
     int base = rp[0], cnt = rp[RB] - base;
-    bool staged = cnt <= STREAM_CAP;
+    bool staged = cnt <= CAP;
     if (staged) {
-        for (int t = tid; t < cnt; t += WG) {
-            int j = base + t;
-            V a = (V)ldg_nt(val + j);
-            if (OP == SpOp::AXPBY) a = alpha * a;
-            prod[t] = a * x[ldg_nt(col + j)];
+        for (int t0 = tid; t0 < cnt; t0 += U * WG) {
+            V a[U], xv[U];
+            int cc[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                int t = t0 + q * WG;
+                a[q] = 0;
+                cc[q] = 0;
+                if (t < cnt) {
+                    a[q] = (V)ldg_nt(val + base + t);
+                    cc[q] = ldg_nt(col + base + t);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) xv[q] = (t0 + q * WG < cnt) ? x[cc[q]] : (V)0;
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                int t = t0 + q * WG;
+                if (t < cnt) prod[t] = (OP == SpOp::AXPBY ? alpha * a[q] : a[q]) * xv[q];
+            }
         }
     }
     __syncthreads();
@@ -202,19 +221,25 @@ static void launchk(Context* c, const SpmvArgs& a) {
     const V* b = (const V*)a.b;
     const V* D = (const V*)a.D;
     V al = (V)a.alpha, be = (V)a.beta;
-    int KP = 2;
-    while (KP < a.k) KP <<= 1;
-#define BCMGX_K(KK)                                                                                                  \
-    case KK:                                                                                                         \
-        k_spmmk<T, V, KK, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, x, y, b, D, \
-                                                                            al, be, a.stop);                       \
+    // columns are processed in chunks of kc: a narrower chunk keeps the
+    // gathered x rows of a stencil neighbourhood inside the L2
+    int kc = a.kchunk > 0 && a.kchunk < a.k ? a.kchunk : a.k;
+    for (int c0 = 0; c0 < a.k; c0 += kc) {
+        int w = a.k - c0 < kc ? a.k - c0 : kc;
+        int KP = 2;
+        while (KP < w) KP <<= 1;
+#define BCMGX_K(KK)                                                                                              \
+    case KK:                                                                                                     \
+        k_spmmk<T, V, KK, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, c0, w, a.row, a.col, val, x, y, \
+                                                                            b, D, al, be, a.stop);               \
         break;
-    switch (KP) {
-        BCMGX_K(2) BCMGX_K(4) BCMGX_K(8) BCMGX_K(16) BCMGX_K(32) BCMGX_K(64)
-    default: throw Error(BCMGX_ERR_ARG, "at most 64 right-hand sides");
-    }
+        switch (KP) {
+            BCMGX_K(2) BCMGX_K(4) BCMGX_K(8) BCMGX_K(16) BCMGX_K(32) BCMGX_K(64)
+        default: throw Error(BCMGX_ERR_ARG, "at most 64 right-hand sides");
+        }
 #undef BCMGX_K
-    BCMGX_LAUNCHED();
+        BCMGX_LAUNCHED();
+    }
 }
 
 template <typename T, typename V>
