@@ -81,6 +81,8 @@ def lib():
         "bcmgx_solve_columns": (C.c_int, [_VP, C.c_int, _I32P, _F64P, _F64P]),
         "bcmgx_dump_info": (C.c_int, [C.c_char_p, C.POINTER(abi.Params), _VP, C.POINTER(abi.SolverOut)]),
         "bcmgx_spmv": (C.c_int, [_VP, _VP, _VP, _VP, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
+        "bcmgx_spmm": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_int]),
         "bcmgx_spgemm": (C.c_int, [_VP, _VP, _VP, C.POINTER(_VP)]),
         "bcmgx_transpose": (C.c_int, [_VP, _VP, C.POINTER(_VP)]),
         "bcmgx_matching": (C.c_int, [_VP, _VP, _F64P, _I32P]),
@@ -300,6 +302,20 @@ class CSR:
         y = np.zeros(self.n) if y is None else np.ascontiguousarray(y, np.float64).copy()
         _check(lib().bcmgx_spmv(self.ctx.h, self.h, _ptr(x), _ptr(y), alpha, beta, op, lanes, MEM_HOST), self.ctx)
         return y
+
+    def spmm(self, X, Y=None, alpha=1.0, beta=0.0, op=1, kchunk=0):
+        """The op of spmv on the k columns of X (n x k) at once."""
+        X = np.asarray(X, np.float64)
+        k = X.shape[1]
+        ld = max(self.n, self.m)  # one leading dimension for both blocks
+        Xp = np.zeros((ld, k), order="F")
+        Xp[: self.m] = X
+        Yp = np.zeros((ld, k), order="F")
+        if Y is not None:
+            Yp[: self.n] = Y
+        _check(lib().bcmgx_spmm(self.ctx.h, self.h, _ptr(Xp), _ptr(Yp), k, ld, alpha, beta, op, kchunk, MEM_HOST),
+               self.ctx)
+        return Yp[: self.n].copy()
 
     def time_spmv(self, precision=0, k=1, lanes=0, reps=20):
         """Average device milliseconds of one y = A x launch (driverSpMV)."""

@@ -28,6 +28,7 @@ static double now_s() {
 // ------------------------------------------------------------ autotune ----
 int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     if (A.n < 65536) return default_lanes(A.nnz, A.n);
+    if (fp32 && !A.valf) throw Error(BCMGX_ERR_STATE, "tune_lanes: FP32 values not built");
     std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d");
     auto it = c->tuned_lanes.find(sig);
     if (it != c->tuned_lanes.end()) return it->second;
@@ -97,6 +98,7 @@ static float time_spmm(Context* c, const DCsr& A, const void* val, int k, int ch
 
 int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
     if (k <= 4 || A.n < 65536) return k;
+    if (fp32 && !A.valf) throw Error(BCMGX_ERR_STATE, "tune_chunk: FP32 values not built");
     std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d") + ":k" + std::to_string(k);
     auto it = c->tuned_lanes.find(sig);
     if (it != c->tuned_lanes.end()) return it->second;

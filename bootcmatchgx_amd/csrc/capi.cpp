@@ -494,7 +494,7 @@ int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, do
     a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.val = M.val.get(); a.nnz = M.nnz;
     a.x = X.get(); a.y = Y.get(); a.b = Y.get(); a.alpha = alpha; a.beta = beta; a.lanes = lanes;
     SpOp o = op == 1 ? SpOp::AX : (op == 2 ? SpOp::RES : SpOp::AXPBY);
-    if (lanes <= 0) a.lanes = tune_lanes(c, M, false);
+    if (lanes == 0) a.lanes = tune_lanes(c, M, false);
     spmv<double, double>(c, o, a);
     hipMemcpyKind kout = mem == BCMGX_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     BCMGX_HIP(hipMemcpyAsync(y, Y.get(), sizeof(double) * M.n, kout, c->stream));
@@ -573,6 +573,29 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
     API_END(ctx)
 }
 
+int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
+               double beta, int op, int kchunk, int mem) {
+    if (!ctx || !A || !x || !y || op < 0 || op > 2 || k < 1 || k > 64 || kchunk < 0) return BCMGX_ERR_ARG;
+    API_BEGIN
+    Context* c = ctx->c;
+    BCMGX_HIP(hipSetDevice(c->device));
+    const DCsr& M = A->A;
+    if (ld < std::max(M.n, M.m)) throw Error(BCMGX_ERR_ARG, "leading dimension smaller than the matrix");
+    DBuf<double> X(c, (size_t)std::max(M.m, 1) * k), Y(c, (size_t)std::max(M.n, 1) * k), tmp;
+    stage_in(c, x, ld, mem, M.m, k, X.get(), tmp);
+    if (op != 1) stage_in(c, y, ld, mem, M.n, k, Y.get(), tmp);
+    SpmvArgs a;
+    a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.val = M.val.get(); a.nnz = M.nnz;
+    a.x = X.get(); a.y = Y.get(); a.b = Y.get(); a.alpha = alpha; a.beta = beta; a.k = k; a.kchunk = kchunk;
+    a.lanes = k == 1 ? tune_lanes(c, M, false) : 0;
+    SpOp o = op == 1 ? SpOp::AX : (op == 2 ? SpOp::RES : SpOp::AXPBY);
+    spmv<double, double>(c, o, a);
+    stage_out(c, Y.get(), y, ld, mem, M.n, k, tmp);
+    c->sync();
+    return BCMGX_OK;
+    API_END(ctx)
+}
+
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms) {
     if (!ctx || !A || !ms || k < 1 || k > 64 || reps < 1 || (precision != 0 && precision != 1)) return BCMGX_ERR_ARG;
     API_BEGIN
@@ -582,14 +605,15 @@ int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int
     size_t nx = (size_t)std::max(M.m, 1) * k, ny = (size_t)std::max(M.n, 1) * k;
     SpmvArgs a;
     a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.nnz = M.nnz; a.k = k;
-    a.lanes = lanes > 0 ? lanes : tune_lanes(c, M, precision == 1);
+    if (precision == 1 && !M.valf) {  // the FP32 copy must exist before any FP32 launch (tuning included)
+        M.valf.reset(c, std::max(M.nnz, 1));
+        convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
+    }
+    if (k == 1) a.lanes = lanes != 0 ? lanes : tune_lanes(c, M, precision == 1);
+    else a.kchunk = lanes > 0 ? lanes : 0;
     DBuf<double> xd, yd;
     DBuf<float> xf, yf;
     if (precision == 1) {
-        if (!M.valf) {
-            M.valf.reset(c, std::max(M.nnz, 1));
-            convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
-        }
         xf.reset(c, nx); yf.reset(c, ny);
         fill<float>(c, xf.get(), (long long)nx, 1.0f);
         a.val = M.valf.get(); a.x = xf.get(); a.y = yf.get();

@@ -44,6 +44,13 @@ __global__ __launch_bounds__(WG) void k_spmv1(int n, const int* __restrict__ row
     }
 }
 
+// order LDS accesses of the lanes of one wave (compiler fence; no hardware barrier)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T, typename V, int KP, SpOp OP>
 __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, int c0, int kc, const int* __restrict__ row,
                                               const int* __restrict__ col, const T* __restrict__ val,
@@ -51,39 +58,48 @@ __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, int c0, int kc, cons
                                               const V* __restrict__ D, V alpha, V beta, const int* __restrict__ stop) {
     if (stop && *stop) return;
     constexpr int RPB = WG / KP;
-    constexpr int U = 8;  // independent (col, val) -> x gathers in flight per lane
+    constexpr int U = KP < 16 ? KP : 16;  // independent x gathers in flight per lane
+    // one (col, val) slot per thread: the KP lanes of a row load KP entries of
+    // it with one coalesced access, then read them back as LDS broadcasts, so
+    // no x gather waits on a matrix load
+    __shared__ int scol[WG];
+    __shared__ T sval[WG];
     int blk = xcd_remap(blockIdx.x, gridDim.x);
-    int r = blk * RPB + (int)threadIdx.x / KP;
-    int cl = (int)threadIdx.x % KP;
-    if (r >= n || cl >= kc) return;
+    int tid = threadIdx.x;
+    int r = blk * RPB + tid / KP;
+    int cl = tid % KP;
+    int g0 = tid - cl;  // first slot of this row's group
+    if (r >= n) return;  // whole group (groups never straddle a wave)
     int c = c0 + cl;  // this lane's column among the k interleaved ones
+    bool active = cl < kc;
     int s = row[r], e = row[r + 1];
     const V* xc = x + c;
     V t = 0;
-    int j = s;
-    for (; j + U <= e; j += U) {
-        // every lane of the group reads the same (col, val): one broadcast access per wave
-        long long cc[U];
-        V a[U], xv[U];
+    for (int base = s; base < e; base += KP) {
+        int j = base + cl;
+        scol[tid] = j < e ? col[j] : 0;
+        sval[tid] = j < e ? val[j] : (T)0;
+        wave_lds_sync();
+        int cnt = e - base < KP ? e - base : KP;
+        if (active) {
+            // U divides KP, so q + u < KP stays inside this group's slots
+            for (int q = 0; q < cnt; q += U) {
+                V xv[U];
 #pragma unroll
-        for (int q = 0; q < U; q++) {
-            cc[q] = col[j + q];
-            a[q] = (V)val[j + q];
+                for (int u = 0; u < U; u++) xv[u] = q + u < cnt ? xc[(long long)scol[g0 + q + u] * k] : (V)0;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (q + u < cnt) {
+                        V a = (V)sval[g0 + q + u];
+                        if (OP == SpOp::AXPBY) t += (alpha * a) * xv[u];
+                        else t += a * xv[u];
+                    }
+                }
+            }
         }
-#pragma unroll
-        for (int q = 0; q < U; q++) xv[q] = xc[cc[q] * k];
-#pragma unroll
-        for (int q = 0; q < U; q++) {
-            if (OP == SpOp::AXPBY) t += (alpha * a[q]) * xv[q];
-            else t += a[q] * xv[q];
-        }
+        wave_lds_sync();
     }
-    for (; j < e; j++) {
-        V a = (V)val[j];
-        long long cc = col[j];
-        if (OP == SpOp::AXPBY) t += (alpha * a) * xc[cc * k];
-        else t += a * xc[cc * k];
-    }
+    if (!active) return;
     long long o = (long long)r * k + c;
     if (OP == SpOp::AX) y[o] = t;
     else if (OP == SpOp::RES) y[o] = b[o] - t;
@@ -245,6 +261,10 @@ static void launchk(Context* c, const SpmvArgs& a) {
 template <typename T, typename V>
 void spmv(Context* c, SpOp op, const SpmvArgs& a) {
     if (a.n <= 0) return;
+    // operands the kernels dereference: a null one is a library bug, reported instead of launched
+    if (!a.row || (a.nnz > 0 && (!a.col || !a.val || !a.x)) || !a.y || a.k < 1 ||
+        ((op == SpOp::RES || op == SpOp::JACOBI) && !a.b) || (op == SpOp::JACOBI && (!a.D || !a.x)))
+        throw Error(BCMGX_ERR_STATE, "spmv: null operand");
     if (a.k == 1) {
         int L = a.lanes != 0 ? a.lanes : default_lanes(a.nnz, a.n);  // < 0: CSR-stream kernel
         switch (op) {

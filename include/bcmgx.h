@@ -213,6 +213,13 @@ int bcmgx_dump_info(const char* path, const bcmgx_params* p, bcmgx_prec prec, co
  * lanes: lanes per row (0 = autotuned choice). */
 int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, double alpha,
                double beta, int op, int lanes, int mem);
+/* The same on k right-hand sides at once (x, y column-major with leading
+ * dimension ld >= n; the device works on k interleaved columns).  No
+ * counterpart in the reference, which multiplies one vector at a time: it is
+ * the multi-vector kernel of the batched solve.  kchunk: columns per launch
+ * (0 = all k).  Each column equals bcmgx_spmv of that column bit for bit. */
+int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
+               double beta, int op, int kchunk, int mem);
 /* C = A*B (replaces nsparse spgemm_csrseg_kernel_hash, kernel_spgemm_hash_d.cu:1196-1247),
  * sorted columns; each C_ij summed over k in ascending order. */
 int bcmgx_spgemm(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix B, bcmgx_matrix* C);
@@ -227,7 +234,9 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
 /* Micro-benchmark of one SpMV/SpMM launch (driverSpMV, src/example/driverSpMV.cu):
  * y = A x on k interleaved columns, FP64 (precision 0) or FP32 matrix and
  * vectors (precision 1); `reps` timed launches on the context stream after one
- * warm-up, bracketed by HIP events.  *ms = average milliseconds per launch. */
+ * warm-up, bracketed by HIP events.  *ms = average milliseconds per launch.
+ * `lanes`: k == 1: lanes per row (0 = autotuned, < 0 = CSR-stream kernel);
+ * k > 1: columns per launch (0 = all k). */
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms);
 /* Sustained device-to-device copy bandwidth (GB/s) over `bytes` bytes. */
 int bcmgx_stream_copy_gbs(bcmgx_context ctx, size_t bytes, int reps, double* gbs);

@@ -76,6 +76,31 @@ def test_spmv_vs_oracle(ctx, name, lanes):
         assert np.all(np.abs(got - ref) <= 1e-14 * scale + 1e-300), (op, lanes)
 
 
+SPMM_CASES = {
+    "lap7_12": CASES["lap7_12"],
+    "lap27_10": CASES["lap27_10"],
+    "rand": CASES["rand"],
+    "long_rows": lambda: O.Csr.from_scipy(rand_spd(300, 0.4, 7)),
+    "rect_empty_rows": lambda: O.Csr.from_scipy(sp.random(260, 190, density=0.05, random_state=8, format="csr")),
+}
+
+
+@pytest.mark.parametrize("name", list(SPMM_CASES))
+@pytest.mark.parametrize("k,kchunk", [(2, 0), (3, 0), (8, 0), (13, 4), (32, 0), (32, 8), (64, 0), (64, 16)])
+def test_spmm_columns_bit_exact(ctx, name, k, kchunk):
+    """Every column of the multi-vector kernel equals the oracle's sequential
+    row sum (same order, no contraction) bit for bit."""
+    oa = SPMM_CASES[name]()
+    A = dev(ctx, oa)
+    rng = np.random.default_rng(k * 7 + kchunk)
+    X, Y = rng.normal(size=(A.m, k)), rng.normal(size=(A.n, k))
+    for op, al, be in [(1, 1.0, 0.0), (2, -1.0, 1.0), (0, 0.7, -1.3)]:
+        got = A.spmm(X, Y, al, be, op, kchunk)
+        for c in range(k):
+            ref = O.spmv(oa, X[:, c].copy(), Y[:, c].copy(), al, be, op)
+            assert np.array_equal(got[:, c], ref), (op, c)
+
+
 # --------------------------------------------------------------- matching
 @pytest.mark.parametrize("name", list(CASES))
 def test_matching_bit_exact(ctx, name):
