@@ -68,6 +68,40 @@ def cpu_baseline(size, nrhs, rtol):
     }
 
 
+def adjoint_timing(ctx, N=128):
+    """Forward solve + adjoint solve through the torch.autograd shim on the 7-point
+    diffusion operator with per-face coefficients U[0.5, 2] (seed 4), loss
+    0.5 ||x - x_target||^2 (x_target ~ N(0,1) seed 5, b ~ N(0,1) seed 6), FP64
+    BCMG hierarchy built once and reused by the adjoint solve.  CPU tensors: the
+    times include the host<->device copies of x, b and the gradients."""
+    import numpy as np
+
+    import bootcmatchgx_amd as bx
+    from bootcmatchgx_amd import autograd as AG
+
+    torch = AG.torch
+    n, row, col, val = bx.diffusion7p(N, N, N, 1, 0.5, 2.0, 4)
+    xt = torch.from_numpy(np.random.default_rng(5).normal(size=n))
+    b = np.random.default_rng(6).normal(size=n)
+    p = bx.Params(solver="FCG", preconditioner="BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, dispnorm=0)
+    t0 = time.perf_counter()
+    system = AG.SPDSystem(n, row, col, val, p, ctx=ctx)
+    t1 = time.perf_counter()
+    v = torch.tensor(val, requires_grad=True)
+    bt = torch.tensor(b, requires_grad=True)
+    x = AG.sparse_solve(v, bt, system)
+    it_f = system.last_info["niter"]
+    L = 0.5 * ((x - xt) ** 2).sum()
+    t2 = time.perf_counter()
+    L.backward()
+    t3 = time.perf_counter()
+    it_a = system.last_info["niter"]
+    system.close()
+    return {"workload": f"7-point diffusion U[0.5,2] {N}^3, FP64 BCMG, rtol 1e-8", "n": n, "nnz": len(col),
+            "setup_s": t1 - t0, "forward_s": t2 - t1, "adjoint_s": t3 - t2, "niter_forward": it_f,
+            "niter_adjoint": it_a}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +111,7 @@ def main():
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 one-at-a-time comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the autograd forward + adjoint timing")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
                     help="only time REPS launches of the FP64 SpMV of the fine matrix (profiling)")
     args = ap.parse_args()
@@ -163,6 +198,8 @@ def main():
     spmm32_ms = A.time_spmv(precision=1, k=k, reps=10)
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
 
+    adjoint = None if args.no_adjoint else adjoint_timing(ctx)
+
     cpu = None if args.no_cpu else cpu_baseline(N, k, rtol)
 
     res = {
@@ -192,6 +229,7 @@ def main():
         "spmv_fp64_gbs": spmv_gbs,
         "spmm_fp32_k32_ms": spmm32_ms,
         "stream_copy_gbs": copy_gbs,
+        "autograd_forward_adjoint": adjoint,
         "roofline": {
             "kernel": "k_spmv1<double,double,L,AX> (fine 27-point matrix, FP64, one vector)",
             "bound": "hbm",

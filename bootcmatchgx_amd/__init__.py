@@ -58,6 +58,7 @@ def lib():
         "bcmgx_csr_destroy": (C.c_int, [_VP]),
         "bcmgx_csr_info": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "bcmgx_csr_copy_to_host": (C.c_int, [_VP, _I32P, _I32P, _F64P]),
+        "bcmgx_csr_update_values": (C.c_int, [_VP, _VP, C.c_int]),
         "bcmgx_gen_laplacian3d": (C.c_int, [C.c_int] * 7 + [C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                              C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_int)),
                                                              C.POINTER(C.POINTER(C.c_double))]),
@@ -282,6 +283,13 @@ class CSR:
         except Exception:
             pass
 
+    def update_values(self, val):
+        """Overwrite the nnz values (same sparsity pattern)."""
+        val = np.ascontiguousarray(val, np.float64)
+        if val.shape != (self.nnz,):
+            raise ValueError(f"expected {self.nnz} values, got shape {val.shape}")
+        _check(lib().bcmgx_csr_update_values(self.h, _ptr(val), MEM_HOST), self.ctx)
+
     def to_host(self):
         row = np.zeros(self.n + 1, np.int32)
         col = np.zeros(self.nnz, np.int32)
@@ -316,6 +324,11 @@ class CSR:
         _check(lib().bcmgx_spmm(self.ctx.h, self.h, _ptr(Xp), _ptr(Yp), k, ld, alpha, beta, op, kchunk, MEM_HOST),
                self.ctx)
         return Yp[: self.n].copy()
+
+    def spmm_device(self, x_ptr, y_ptr, k, ld, alpha=1.0, beta=0.0, op=1, kchunk=0):
+        """spmm on column-major device blocks (leading dimension ld)."""
+        _check(lib().bcmgx_spmm(self.ctx.h, self.h, x_ptr, y_ptr, k, ld, alpha, beta, op, kchunk, MEM_DEVICE),
+               self.ctx)
 
     def time_spmv(self, precision=0, k=1, lanes=0, reps=20):
         """Average device milliseconds of one y = A x launch (driverSpMV)."""

@@ -238,6 +238,24 @@ int bcmgx_csr_info(bcmgx_matrix A, int* n, int* m, int* nnz) {
     return BCMGX_OK;
 }
 
+int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem) {
+    if (!A || (A->A.nnz > 0 && !val)) return BCMGX_ERR_ARG;
+    if (!A->ctx) return BCMGX_ERR_STATE;
+    API_BEGIN
+    Context* c = A->ctx->c;
+    BCMGX_HIP(hipSetDevice(c->device));
+    DCsr& M = A->A;
+    if (M.nnz > 0) {
+        hipMemcpyKind k = mem == BCMGX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        BCMGX_HIP(hipMemcpyAsync(M.val.get(), val, sizeof(double) * (size_t)M.nnz, k, c->stream));
+        // an FP32 copy (built by an FP32 hierarchy on this matrix) follows the new values
+        if (M.valf) convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
+    }
+    c->sync();
+    return BCMGX_OK;
+    API_END(A->ctx)
+}
+
 int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val) {
     if (!A) return BCMGX_ERR_ARG;
     if (!A->ctx) return BCMGX_ERR_STATE;

@@ -126,6 +126,12 @@ int bcmgx_csr_destroy(bcmgx_matrix A);
 int bcmgx_csr_info(bcmgx_matrix A, int* n, int* m, int* nnz);
 /* copy CSR arrays back to host buffers (row: n+1, col/val: nnz) */
 int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val);
+/* Overwrite the nnz values of A (same pattern).  No counterpart in the
+ * reference, whose matrices are immutable after copyToDevice (CSR.h:347);
+ * used by the autograd shim when the values it differentiates change.  A
+ * preconditioner set up on A keeps its hierarchy (its level 0 is A itself,
+ * so it sees the new values; coarse levels are not rebuilt). */
+int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem);
 
 /* Host-side problem generators: src/generator/laplacian.cu:119-334 (7-point),
  * :336-777 (27-point), block ordering internal_index (:14-25).  Output arrays

@@ -1,0 +1,128 @@
+"""Full-size runs of the solve path, checked through properties that do not
+need the oracle at these sizes: the true residual ||b - A x|| / ||b|| of
+every column (recomputed by a separate multi-vector product, not the
+solver's recurrence) is at most the requested 1e-8, mixed precision agrees
+with the FP64 hierarchy, batched columns agree with one-at-a-time solves,
+and b = A x* recovers x*.
+
+Workloads: the 27-point Laplacian on 128^3 (2,097,152 rows, 55,742,968
+nonzeros) with x* ~ U[-1,1] (numpy PCG64 seed 1) and with k = 1, 8, 32, 64
+right-hand sides N(0,1) (seed 3); the 7-point diffusion operator on 160^3
+with per-face conductivities exp(2 g), g ~ N(0,1) (seed 2)."""
+import numpy as np
+import pytest
+
+import bootcmatchgx_amd as bx
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-8
+
+
+def params(prec="BCMG", precision="FP64", **kw):
+    return bx.Params(solver="FCG", preconditioner=prec, rtol=RTOL, prerelax_sweeps=1, postrelax_sweeps=1,
+                     dispnorm=0, precision=precision, **kw)
+
+
+def true_rel_res(ctx, A, B, X, k):
+    """||b_c - A x_c|| / ||b_c|| per column, from device-resident blocks."""
+    n = A.n
+    R = bx.DeviceArray(ctx, (n, k))
+    R.copy_from(B)
+    A.spmm_device(X.ptr, R.ptr, k, n, -1.0, 1.0, 2)
+    r = R.download().reshape((n, k), order="F")
+    b = B.download().reshape((n, k), order="F")
+    R.close()
+    return np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0)
+
+
+@pytest.fixture(scope="module")
+def lap27(ctx):
+    n, row, col, val = bx.laplacian3d(27, 128, 128, 128)
+    A = bx.CSR(ctx, n, n, row, col, val)
+    yield A
+    A.close()
+
+
+@pytest.mark.parametrize("prec", ["BCMG", "AFSAI"])
+def test_c1_fp64_recovers_manufactured_solution(ctx, lap27, prec):
+    A = lap27
+    n = A.n
+    xs = np.random.default_rng(1).uniform(-1.0, 1.0, size=n)
+    XS, B, X = bx.DeviceArray(ctx, (n,)), bx.DeviceArray(ctx, (n,)), bx.DeviceArray(ctx, (n,))
+    XS.upload(xs)
+    A.spmm_device(XS.ptr, B.ptr, 1, n)  # b = A x*
+    X.upload(np.zeros(n))
+    p = params(prec)
+    P = bx.prec_setup(ctx, A, p)
+    out = bx.solve_device(ctx, A, B.ptr, X.ptr, 1, p, P)
+    P.finalize()
+    rr = true_rel_res(ctx, A, B, X, 1)[0]
+    err = np.linalg.norm(X.download() - xs) / np.linalg.norm(xs)
+    print(f"C1 {prec}: niter {out['niter']} true rel res {rr:.3e} error vs x* {err:.3e}")
+    assert out["retv"] == 0
+    assert rr <= RTOL * 1.01
+    # ||x - x*|| / ||x*|| <= cond(A) ||r|| / ||b||.  The 27-point operator's
+    # eigenvalues are 27 - prod_d (1 + 2 cos(pi j_d / 129)), so
+    # lambda_min = 27 - (1 + 2 cos(pi / 129))^3, and lambda_max <= 52 (Gershgorin)
+    lam_min = 27.0 - (1.0 + 2.0 * np.cos(np.pi / 129.0)) ** 3
+    assert err <= (52.0 / lam_min) * rr
+
+
+@pytest.mark.parametrize("k", [1, 8, 32, 64])
+def test_c3_batched_fp32_hierarchy_converges_every_column(ctx, lap27, k):
+    A = lap27
+    n = A.n
+    Bh = np.random.default_rng(3).normal(size=(n, k))
+    B, X = bx.DeviceArray(ctx, (n, k)), bx.DeviceArray(ctx, (n, k))
+    B.upload(Bh.ravel(order="F"))
+    X.upload(np.zeros(n * k))
+    p = params(precision="FP32")
+    P = bx.prec_setup(ctx, A, p)
+    out = bx.solve_device(ctx, A, B.ptr, X.ptr, k, p, P)
+    rr = true_rel_res(ctx, A, B, X, k)
+    its = out["niter_cols"]
+    # one-at-a-time FP32 solves of the first and last column with the same hierarchy
+    for c in sorted({0, k - 1}):
+        Xc = bx.DeviceArray(ctx, (n,))
+        Xc.upload(np.zeros(n))
+        o1 = bx.solve_device(ctx, A, B.at(c * n), Xc.ptr, 1, p, P)
+        assert abs(o1["niter"] - its[c]) <= 1, (c, o1["niter"], its[c])
+        xc = Xc.download()
+        xb = X.download().reshape((n, k), order="F")[:, c]
+        assert np.linalg.norm(xb - xc) / np.linalg.norm(xc) < 1e-6
+        Xc.close()
+    P.finalize()
+    print(f"C3 k={k}: niter {min(its)}..{max(its)}, max true rel res {rr.max():.3e}")
+    assert out["retv"] == 0
+    assert np.all(rr <= RTOL * 1.01)
+
+
+def test_c2_fp32_hierarchy_matches_fp64(ctx):
+    n, row, col, val = bx.diffusion7p(160, 160, 160, 0, 2.0, 0.0, 2)
+    A = bx.CSR(ctx, n, n, row, col, val)
+    del row, col, val
+    b = np.random.default_rng(2).uniform(-1.0, 1.0, size=n)
+    B = bx.DeviceArray(ctx, (n,))
+    B.upload(b)
+    xs = {}
+    for precision in ("FP64", "FP32"):
+        X = bx.DeviceArray(ctx, (n,))
+        X.upload(np.zeros(n))
+        p = params(precision=precision)
+        P = bx.prec_setup(ctx, A, p)
+        out = bx.solve_device(ctx, A, B.ptr, X.ptr, 1, p, P)
+        P.finalize()
+        rr = true_rel_res(ctx, A, B, X, 1)[0]
+        print(f"C2 {precision}: niter {out['niter']} true rel res {rr:.3e}")
+        assert out["retv"] == 0
+        assert rr <= RTOL * 1.01
+        xs[precision] = X.download()
+        X.close()
+    err = np.linalg.norm(xs["FP32"] - xs["FP64"]) / np.linalg.norm(xs["FP64"])
+    print(f"C2 FP32 vs FP64 hierarchy: relative difference {err:.3e}")
+    # both solutions have residual <= 1e-8 ||b||, so they differ by at most
+    # 2e-8 cond(A); cond(A) is not computed here, the bound below is a sanity
+    # limit and the measured difference is printed
+    assert err < 1e-3
+    A.close()
