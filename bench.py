@@ -33,6 +33,20 @@ def spmv_fp64_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
+def pmc_traffic(name="spmv_fp64_lap27_128.txt"):
+    """HBM bytes per launch of the fine FP64 SpMV from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(size, nrhs, rtol):
     """The oracle (test infrastructure) timed on the host: FP64 setup + one solve."""
     import numpy as np
@@ -231,13 +245,14 @@ def main():
         "stream_copy_gbs": copy_gbs,
         "autograd_forward_adjoint": adjoint,
         "roofline": {
-            "kernel": "k_spmv1<double,double,L,AX> (fine 27-point matrix, FP64, one vector)",
+            "kernel": "k_spmv_stream<double,double,TPR,AX> (autotuned CSR-stream SpMV, fine 27-point matrix, FP64)",
             "bound": "hbm",
             "achieved": spmv_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": spmv_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": pmc_traffic() if N == 128 else None,
+            "traffic_source": "profiles/spmv_fp64_lap27_128.txt (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "algorithmic_bytes": spmv_bytes,
             "avg_launch_ms": spmv_ms,
         },

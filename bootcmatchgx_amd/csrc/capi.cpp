@@ -400,6 +400,8 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     if (!ctx || !A || !p || !b || !x || nrhs < 1 || nrhs > 64) return BCMGX_ERR_ARG;
     if (A->ctx != ctx || (prec && prec->ctx != ctx)) return BCMGX_ERR_STATE;
     API_BEGIN
+    // solve() dispatches on the solver type (src/solver/solve.cu:19-40); only FCG is on this path
+    BCMGX_CHECK(p->solver == BCMGX_SOLVER_FCG, BCMGX_ERR_UNSUPPORTED, "only the FCG solver is supported");
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
     int n = A->A.n;

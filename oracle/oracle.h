@@ -5,6 +5,22 @@
  * restatement of BootCMatchGX's single-GPU solve path, used as the checker
  * for the HIP product and as the timed CPU baseline.  Nothing in the product
  * (bootcmatchgx_amd/) may link or call it.
+ *
+ * Pinned by the reference's fixtures and by mathematics (tests/test_oracle.py):
+ * generators (MATLAB .mtx fixtures, closed forms), MTX reader (bcsstk09),
+ * spmv/spgemm/transpose (dense products), matching weights (2x2 quadratic-form
+ * identity), suitor (independent greedy matching + invariants), make_p (known
+ * answer), hierarchy (dense P^T A P, coarse-size truncation), whole solves of
+ * every preconditioner (the reference's solution fixtures at 1e-5), AFSAI
+ * (triangularity, unit diagonal of G A G^T).
+ *
+ * Parity unpinned: the reference binary's own FCG iteration counts and level
+ * shapes (none published; counts are checked against this restatement only);
+ * its atomic-order singleton numbering (canonicalised here to ascending
+ * index); its lock-based Suitor reaching the greedy matching; nsparse/cuBLAS
+ * round-off; the last ulp of log() between glibc and the GPU; AFSAI argmax
+ * ties (resolved here to the smallest column).  The diffusion generator is
+ * build-defined and has no reference counterpart.
  */
 #ifndef BCMGX_ORACLE_H
 #define BCMGX_ORACLE_H

@@ -363,3 +363,15 @@ def test_sparse_outer_grad(ctx):
     r, c, _ = oa.arrays()
     rows = np.repeat(np.arange(A.n), np.diff(r))
     assert np.allclose(g, -np.sum(lam[rows] * x[c], axis=1), rtol=1e-14, atol=1e-14)
+
+
+def test_non_fcg_solver_is_rejected(ctx):
+    oa = CASES["lap7_12"]()
+    A = dev(ctx, oa)
+    p = params("BCMG")
+    P = bx.prec_setup(ctx, A, p)
+    p.set("solver", "CGHS")
+    with pytest.raises(bx.BcmgxError) as e:
+        bx.solve(ctx, A, np.ones(A.n), None, p, P)
+    assert e.value.status == -6 and "FCG" in str(e.value)
+    P.finalize()

@@ -144,22 +144,31 @@ def test_suitor_equals_greedy_and_invariants(name, make):
                 assert M[c[j]] >= 0, "positive edge between two unmatched vertices"
 
 
-def test_matching_weights_symmetric_and_formula():
+def test_matching_weights_quadratic_form_identity():
+    """|AH_ij| is the energy of the pair vector v = (w_i, -w_j) under the 2x2
+    principal submatrix A2 of (i, j), relative to the diagonal energy of
+    u = (w_i, w_j): |AH_ij| = |v' A2 v| / (u' diag(A2) u).  Computed here with
+    numpy matrix products, independent of the oracle's formula code; W must be
+    its log wherever it is above the clamp, and W is symmetric."""
     S = rand_spd(40, 0.15, 7)
     A = O.Csr.from_scipy(S)
     w = np.random.default_rng(3).uniform(0.5, 2, 40)
     r, c, v = O.matching_weights(A, w)
     W = sp.csr_matrix((v, c, r), shape=(40, 40))
     assert abs(W - W.T).max() == 0
-    D = S.diagonal()
-    Cc = D * w * w
-    i, j = 3, None
-    row = S.getrow(i)
-    for jj, a in zip(row.indices, row.data):
-        if jj == i:
-            continue
-        ah = 1 - 2 * a * w[i] * w[jj] / (Cc[i] + Cc[jj])
-        assert abs(W[i, jj] - np.log(abs(ah))) < 1e-13
+    Sd = S.toarray()
+    checked = 0
+    for i in range(40):
+        for j in np.nonzero(Sd[i])[0]:
+            if j == i:
+                continue
+            A2 = Sd[np.ix_([i, j], [i, j])]
+            vv = np.array([w[i], -w[j]])
+            uu = np.array([w[i], w[j]])
+            ah = abs(vv @ A2 @ vv) / (uu @ np.diag(np.diag(A2)) @ uu)
+            assert np.isclose(np.exp(W[i, j]), ah, rtol=1e-12, atol=0), (i, j)
+            checked += 1
+    assert checked > 100
 
 
 def test_tied_laplacian_pairs_along_x():
