@@ -233,7 +233,33 @@ static void make_fp32_copies(Context* c, Hierarchy& H) {
 }
 
 // ------------------------------------------------------------ prec ----
+struct SolveWork {
+    int k = 0, n = 0;
+    DBuf<double> B, X, r, v, w, d, q;
+    DBuf<double> partials;
+    DBuf<FcgState> st;
+    DBuf<int> flags;  // [0] all_done, [1] columns done
+    FcgBufs fb;
+    FcgEntry entry;     // level-0 entry of the FP32 cycle filled by the updates
+    bool fused = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int lanesA = 0;
+    int chunkA = 0;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~SolveWork() { drop_graph(); }
+};
+
 Prec::~Prec() {}
+
+void Prec::drop_graphs() {
+    for (auto& s : solves) s->drop_graph();
+}
 
 void Prec::ensure_work(int k) {
     if (ptype == BCMGX_PREC_BCMG && k > 1 && k != tuned_k) {
@@ -244,6 +270,9 @@ void Prec::ensure_work(int k) {
     }
     if (k <= kcap) return;
     kcap = k;
+    // the cycle buffers are about to be reallocated: graphs captured by earlier
+    // solves hold their addresses, so they are dropped and captured again
+    drop_graphs();
     if (ptype == BCMGX_PREC_BCMG) {
         int nl = H.levels();
         if (p.precision == BCMGX_PRECISION_FP32) {
@@ -322,7 +351,7 @@ static const V* lvl_pval(const Level& l) {
 // buffer (one of W[l].X / W[l].X2) or nullptr for zero.  Returns the buffer
 // holding the level's result.
 template <typename V>
-static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin) {
+static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep_done = false) {
     Context* c = R.c;
     Prec* pr = R.pr;
     const bcmgx_params& p = pr->p;
@@ -351,8 +380,16 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin) {
         }
     }
     int coeff = p.cycle_type == BCMGX_CYCLE_VARIABLE_V ? (1 << l) : 1;
-    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff, xzero, k, R.stop,
-                           lv.chunkA);
+    if (first_sweep_done) {
+        // w.X already holds the first sweep from zero, F / M
+        cur = w.X.get();
+        oth = w.X2.get();
+        cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff - 1, false, k, R.stop,
+                               lv.chunkA);
+    } else {
+        cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff, xzero, k, R.stop,
+                               lv.chunkA);
+    }
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
     // Res = F - A x ; F_{l+1} = R Res
     {
@@ -385,7 +422,17 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin) {
     return cur;
 }
 
-void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop) {
+bool Prec::fused_entry(int k, FcgEntry* e) {
+    if (ptype != BCMGX_PREC_BCMG || p.precision != BCMGX_PRECISION_FP32 || H.levels() < 2 || p.prerelax_sweeps < 1)
+        return false;
+    ensure_work(k);
+    e->F = wf[0].F.get();
+    e->X = wf[0].X.get();
+    e->M = H.L[0]->Mf.get();
+    return true;
+}
+
+void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready) {
     ensure_work(k);
     Context* c = ctx;
     long long nk = (long long)A->n * k;
@@ -412,13 +459,19 @@ void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop
     case BCMGX_PREC_BCMG: {  // bcmg_apply (bcmg.cu:51-90)
         if (p.precision == BCMGX_PRECISION_FP32) {
             CycleRun<float> R{c, this, &wf, k, stop};
-            convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
-            float* xin = nullptr;
-            if (!x_zero) {
-                convert_stop<double, float>(c, u, wf[0].X.get(), nk, stop);
-                xin = wf[0].X.get();
+            float* res;
+            if (entry_ready) {
+                BCMGX_CHECK(x_zero && H.levels() >= 2 && p.prerelax_sweeps >= 1, BCMGX_ERR_STATE, "level-0 entry not applicable");
+                res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true);
+            } else {
+                convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
+                float* xin = nullptr;
+                if (!x_zero) {
+                    convert_stop<double, float>(c, u, wf[0].X.get(), nk, stop);
+                    xin = wf[0].X.get();
+                }
+                res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin);
             }
-            float* res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin);
             convert_stop<float, double>(c, res, u, nk, stop);
         } else {
             CycleRun<double> R{c, this, &wd, k, stop};
@@ -483,22 +536,6 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
 }
 
 // ---------------------------------------------------------------- FCG ----
-struct SolveWork {
-    int k = 0, n = 0;
-    DBuf<double> B, X, r, v, w, d, q;
-    DBuf<double> partials;
-    DBuf<FcgState> st;
-    DBuf<int> flags;  // [0] all_done, [1] columns done
-    FcgBufs fb;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int lanesA = 0;
-    int chunkA = 0;
-    ~SolveWork() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-    }
-};
 
 static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, int idx) {
     double* P0 = idx == 0 ? s.v.get() : s.d.get();
@@ -506,13 +543,13 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     double* P1 = idx == 0 ? s.d.get() : s.v.get();
     double* W1 = idx == 0 ? s.q.get() : s.w.get();
     const int* stop = s.flags.get();
-    prec->apply(s.r.get(), P0, s.k, true, stop);  // P0 zero-filled in the reference (FCG.cu:106,119)
+    prec->apply(s.r.get(), P0, s.k, true, stop, s.fused);  // P0 zero-filled in the reference (FCG.cu:106,119)
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
     a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
     spmv<double, double>(c, SpOp::AX, a);
     fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1);
-    fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get());
+    fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get(), s.entry);
     fcg_finish_iteration(c, s.fb);
 }
 
@@ -549,6 +586,8 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     s.fb.rtol = p.rtol;
     s.fb.itnlim = p.itnlim;
     prec->ensure_work(k);
+    s.entry = FcgEntry();
+    s.fused = prec->fused_entry(k, &s.entry);
     long long nk = (long long)n * k;
     copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
     copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
@@ -569,7 +608,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         spmv<double, double>(c, SpOp::AX, a);  // w = A v
     }
     fcg_init_dots(c, s.fb, s.r.get(), s.v.get(), s.w.get());
-    fcg_init_update(c, s.fb, s.X.get(), s.r.get(), s.v.get(), s.w.get(), s.d.get(), s.q.get());
+    fcg_init_update(c, s.fb, s.X.get(), s.r.get(), s.v.get(), s.w.get(), s.d.get(), s.q.get(), s.entry);
     // ---- iterations, two at a time (the buffer roles alternate)
     int check = std::max(2, p.check_every + (p.check_every & 1));
     bool use_graph = p.use_graph != 0;

@@ -65,8 +65,14 @@ struct Prec {
 
     ~Prec();
     void ensure_work(int k);
-    // u <- B r for k interleaved columns; `x_zero`: u holds zeros on entry (the FCG case)
-    void apply(const double* r, double* u, int k, bool x_zero, const int* stop);
+    void drop_graphs();
+    // u <- B r for k interleaved columns; `x_zero`: u holds zeros on entry (the FCG case).
+    // `entry_ready`: the level-0 entry of the FP32 cycle (see fused_entry) already
+    // holds F = (float) r and the first sweep X = F / M.
+    void apply(const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready = false);
+    // the level-0 buffers an FCG update can fill for the next apply (FP32 BCMG
+    // with at least two levels and a pre-smoothing sweep); false otherwise
+    bool fused_entry(int k, FcgEntry* e);
 };
 
 std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& p);

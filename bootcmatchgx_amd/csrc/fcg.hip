@@ -145,24 +145,33 @@ void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* 
 __global__ __launch_bounds__(WG) void k_init_update(int n, int k, const FcgState* __restrict__ st, double* __restrict__ x,
                                                     double* __restrict__ r, const double* __restrict__ v,
                                                     const double* __restrict__ w, double* __restrict__ d,
-                                                    double* __restrict__ q) {
+                                                    double* __restrict__ q, FcgEntry e) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * k) return;
-    int c = (int)(t % k);
+    int i = (int)(t / k);
+    int c = (int)(t - (long long)i * k);
     double th = st[c].theta;
     double vv = v[t], ww = w[t];
+    double rr = r[t];
     if (!st[c].done) {
         x[t] += th * vv;      // my_axpby(v, x, theta, 1): y += a x  (basic.cu:114-130)
-        r[t] += -th * ww;     // my_axpby(w, r, -theta, 1)
+        rr += -th * ww;       // my_axpby(w, r, -theta, 1)
+        r[t] = rr;
     }
     d[t] = vv;                // d = clone(v), q = clone(w)  (FCG.cu:98-99)
     q[t] = ww;
+    if (e.F) {
+        float f = (float)rr;
+        e.F[t] = f;
+        e.X[t] = f / e.M[i];
+    }
 }
 
-void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q) {
+void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q,
+                     const FcgEntry& e) {
     long long nk = (long long)f.n * f.k;
     if (!nk) return;
-    k_init_update<<<grid_for(nk, WG), WG, 0, c->stream>>>(f.n, f.k, f.st, x, r, v, w, d, q);
+    k_init_update<<<grid_for(nk, WG), WG, 0, c->stream>>>(f.n, f.k, f.st, x, r, v, w, d, q, e);
     BCMGX_LAUNCHED();
 }
 
@@ -199,7 +208,8 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
                                                double* __restrict__ p0, const double* __restrict__ p1,
                                                double* __restrict__ w0, const double* __restrict__ w1,
                                                double* __restrict__ x, double* __restrict__ r,
-                                               double* __restrict__ partials, const int* __restrict__ all_done) {
+                                               double* __restrict__ partials, const int* __restrict__ all_done,
+                                               FcgEntry e) {
     if (*all_done) return;
     __shared__ double s[WG];
     int tid = threadIdx.x;
@@ -224,6 +234,11 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
                 rr = r[o];
             }
             acc += rr * rr;
+            if (e.F) {
+                float f = (float)rr;
+                e.F[o] = f;
+                e.X[o] = f / e.M[i];
+            }
         }
     }
     s[tid] = acc;
@@ -235,9 +250,10 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
     if (tid < k) partials[((long long)blockIdx.x * k + tid) * 3] = s[tid];
 }
 
-void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r) {
+void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r,
+                const FcgEntry& e) {
     int KP = kpad(f.k);
-    k_update<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p0, p1, w0, w1, x, r, f.partials, f.all_done);
+    k_update<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p0, p1, w0, w1, x, r, f.partials, f.all_done, e);
     BCMGX_LAUNCHED();
 }
 

@@ -80,14 +80,24 @@ struct FcgBufs {
 int fcg_partial_blocks(int n);
 // ||v||^2 partials of each column -> st[c].l2 = sqrt(sum); mode 0: delta0 = l2 (initial)
 void fcg_norm(Context* c, const FcgBufs& f, const double* v, int mode);
+// Optional level-0 entry of an FP32 V-cycle written by the FCG updates: with
+// the new residual r, F = (float) r and X = F / M (the first l1-Jacobi sweep
+// from a zero guess), so the next preconditioner application starts there.
+struct FcgEntry {
+    float* F = nullptr;
+    float* X = nullptr;
+    const float* M = nullptr;
+};
 // alpha = r.v, beta = w.v -> delta = beta, theta = alpha / delta  (FCG.cu:56-86)
 void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* v, const double* w);
 // x += theta v; r -= theta w  (FCG.cu:88-89); d = v, q = w
-void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q);
+void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q,
+                     const FcgEntry& e = FcgEntry());
 // (alpha, beta, gamma) = (r.p0, w0.p0, w1.p0), then theta, delta, theta2 (FCG.cu:113-136)
 void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1);
 // double_merged_axpy x2 (FCG.cu:138-144) fused with the ||r||^2 partials
-void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r);
+void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r,
+                const FcgEntry& e = FcgEntry());
 // ||r|| -> convergence test and iteration count (FCG.cu:146-156)
 void fcg_finish_iteration(Context* c, const FcgBufs& f);
 

@@ -284,11 +284,15 @@ __global__ void k_make_wrow(int n, const int* __restrict__ row, int* __restrict_
 template <int G>
 static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, const double* Wval, const int* tkey,
                           const int* inv, int* S) {
-    const int maxsteps = 8;
+    // proposals per vertex per launch: short while most vertices are pending
+    // (the launch is throughput-bound), longer once only chains remain (each
+    // launch is then latency-bound and fewer launches finish them sooner)
+    auto maxsteps_of = [](int round) { return round < 8 ? 8 : round < 16 ? 16 : round < 24 ? 32 : 64; };
     DBuf<int> la(c, n), lb(c, n), cnt(c, 2);
     fill<int>(c, cnt.get(), 2, 0);
     unsigned grid = (unsigned)std::min<long long>(grid_for(n, WG / G), (long long)c->num_cus * 16);
-    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, nullptr, nullptr, la.get(), cnt.get(), maxsteps);
+    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, nullptr, nullptr, la.get(), cnt.get(),
+                                             maxsteps_of(0));
     BCMGX_LAUNCHED();
     int* lists[2] = {la.get(), lb.get()};
     int cur = 0;
@@ -296,7 +300,7 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, c
         // lists[cur] / cnt[cur] hold the pending vertices; clear the other counter and run a round
         fill<int>(c, cnt.get() + (cur ^ 1), 1, 0);
         k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, lists[cur], cnt.get() + cur,
-                                                 lists[cur ^ 1], cnt.get() + (cur ^ 1), maxsteps);
+                                                 lists[cur ^ 1], cnt.get() + (cur ^ 1), maxsteps_of(round));
         BCMGX_LAUNCHED();
         cur ^= 1;
         if (round % 8 == 0 && c->read1(cnt.get() + cur) == 0) break;

@@ -107,9 +107,75 @@ __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, int c0, int kc, cons
     else y[o] = ((-t + b[o]) / D[r]) + x[o];
 }
 
+// k-vector SpMM with CPL consecutive columns per lane (one 8-16 byte vector
+// gather per nonzero): a group of KP lanes owns a row, stages M*KP (col, val)
+// entries of it in LDS with coalesced loads, then every lane gathers its CPL
+// columns of the x rows, U gathers in flight.  Each column's sum runs over the
+// row in order, as in k_spmmk, so the results are the same bits.  Needs
+// k % CPL == 0 and KP * CPL >= k; lanes past k / CPL idle.
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP>
+__global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
+                                              const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
+                                              const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
+                                              const int* __restrict__ stop) {
+    typedef V VV __attribute__((ext_vector_type(CPL)));
+    if (stop && *stop) return;
+    constexpr int RPB = WG / KP;
+    constexpr int E = KP * M;
+    constexpr int U = 16;
+    __shared__ int scol[WG * M];
+    __shared__ T sval[WG * M];
+    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int tid = threadIdx.x;
+    int cl = tid % KP;
+    int g = tid / KP;
+    int r = blk * RPB + g;
+    if (r >= n) return;  // whole group (groups never straddle a wave)
+    bool active = cl * CPL < k;
+    int s = row[r], e = row[r + 1];
+    int* sc = scol + g * E;
+    T* sv = sval + g * E;
+    const V* xc = x + cl * CPL;
+    VV t = (VV)(V)0;
+    for (int base = s; base < e; base += E) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            int j = base + m * KP + cl;
+            sc[m * KP + cl] = j < e ? col[j] : 0;
+            sv[m * KP + cl] = j < e ? val[j] : (T)0;
+        }
+        wave_lds_sync();
+        int cnt = e - base < E ? e - base : E;
+        if (active) {
+            for (int q = 0; q < cnt; q += U) {
+                VV xv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    xv[u] = q + u < cnt ? *(const VV*)(xc + (long long)sc[q + u] * k) : (VV)(V)0;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (q + u < cnt) {
+                        V a = (V)sv[q + u];
+                        if (OP == SpOp::AXPBY) t += (alpha * a) * xv[u];
+                        else t += a * xv[u];
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+    }
+    if (!active) return;
+    long long o = (long long)r * k + cl * CPL;
+    VV* yo = (VV*)(y + o);
+    if (OP == SpOp::AX) *yo = t;
+    else if (OP == SpOp::RES) *yo = *(const VV*)(b + o) - t;
+    else if (OP == SpOp::AXPBY) *yo = t + beta * *yo;
+    else *yo = ((-t + *(const VV*)(b + o)) / D[r]) + *(const VV*)(x + o);
+}
+
 // CSR-stream SpMV for one vector: a workgroup owns RB = 256/TPR consecutive
 // rows; the products of their (contiguous) nonzeros are formed with fully
-// coalesced streaming loads into LDS (four independent val/col/x loads per
+// coalesced streaming loads into LDS (eight independent val/col/x loads per
 // thread in flight), then TPR lanes reduce each row from LDS.  The LDS tile
 // holds 32 nonzeros per row on average (so a block stays small enough for
 // full occupancy); blocks with more reduce straight from global memory.
@@ -121,18 +187,18 @@ __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict
     if (stop && *stop) return;
     constexpr int RB = WG / TPR;
     constexpr int CAP = RB * 32 < 4096 ? RB * 32 : 4096;
-    constexpr int U = 4;
+    constexpr int U = 8;
     __shared__ V prod[CAP];
     __shared__ int rp[RB + 1];
     int blk = xcd_remap(blockIdx.x, gridDim.x);
     int r0 = blk * RB;
     int tid = threadIdx.x;
-    for (int t = tid; t <= RB; t += WG) {
-        int rr = r0 + t;
-        rp[t] = row[rr < n ? rr : n];
-    }
-    __syncthreads();
-    int base = rp[0], cnt = rp[RB] - base;
+    // the block's nonzero range from two uniform (scalar) loads; the per-row
+    // pointers of the reduction load alongside the val/col stream below
+    int rend = r0 + RB < n ? r0 + RB : n;
+    int base = row[r0], cnt = row[rend] - base;
+    int myrp = 0;
+    if (tid < RB) myrp = row[r0 + tid < n ? r0 + tid : n];
     bool staged = cnt <= CAP;
     if (staged) {
         for (int t0 = tid; t0 < cnt; t0 += U * WG) {
@@ -157,6 +223,8 @@ __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict
             }
         }
     }
+    if (tid < RB) rp[tid] = myrp;
+    if (tid == 0) rp[RB] = base + cnt;  // = row[rend]
     __syncthreads();
     int lr = tid / TPR, sub = tid % TPR;
     int r = r0 + lr;
@@ -240,6 +308,26 @@ static void launchk(Context* c, const SpmvArgs& a) {
     // columns are processed in chunks of kc: a narrower chunk keeps the
     // gathered x rows of a stencil neighbourhood inside the L2
     int kc = a.kchunk > 0 && a.kchunk < a.k ? a.kchunk : a.k;
+    // whole rows of x as 16-byte vectors per lane: CPL = 16 / sizeof(V) columns
+    constexpr int CPL = 16 / (int)sizeof(V);
+    bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b) & 15) == 0;
+    if (kc == a.k && a.k % CPL == 0 && a.k >= 2 * CPL && aligned) {
+        int lanes = a.k / CPL;
+        int KP = 2;
+        while (KP < lanes) KP <<= 1;
+#define BCMGX_V(KK, MM)                                                                                            \
+    case KK:                                                                                                       \
+        k_spmmv<T, V, KK, CPL, MM, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, x, \
+                                                                                     y, b, D, al, be, a.stop);     \
+        break;
+        switch (KP) {
+            BCMGX_V(2, 8) BCMGX_V(4, 8) BCMGX_V(8, 4) BCMGX_V(16, 2) BCMGX_V(32, 1)
+        default: throw Error(BCMGX_ERR_ARG, "at most 64 right-hand sides");
+        }
+#undef BCMGX_V
+        BCMGX_LAUNCHED();
+        return;
+    }
     for (int c0 = 0; c0 < a.k; c0 += kc) {
         int w = a.k - c0 < kc ? a.k - c0 : kc;
         int KP = 2;

@@ -325,6 +325,29 @@ def test_batched_columns_match_single_solves(ctx, precision):
         assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, j]) * 1.0000001
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_solves_after_wider_batch_reuse_no_stale_graph(ctx, precision):
+    """A single-column solve, then a wider batch on the same preconditioner
+    (its cycle buffers grow), then the single-column solve again: the replayed
+    iteration graph must use the new buffers and give the same bits."""
+    oa = O.laplacian3d(27, 20, 20, 20)
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    p = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, precision=precision)
+    P = bx.prec_setup(ctx, A, p)
+    b = np.random.default_rng(11).normal(size=n)
+    x1, o1 = bx.solve(ctx, A, b, None, p, P)
+    B = np.random.default_rng(12).normal(size=(n, 8))
+    X8, o8 = bx.solve(ctx, A, B, None, p, P)
+    x2, o2 = bx.solve(ctx, A, b, None, p, P)
+    assert o1["niter"] == o2["niter"]
+    assert np.array_equal(x1, x2)
+    for j in range(8):
+        r = B[:, j] - O.spmv(oa, X8[:, j])
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, j]) * 1.0000001
+    P.finalize()
+
+
 def test_graph_replay_equals_eager(ctx):
     oa = O.laplacian3d(27, 16, 16, 16)
     b = O.spmv(oa, np.random.default_rng(9).uniform(-1, 1, 4096))
