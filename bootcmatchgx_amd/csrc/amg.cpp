@@ -246,6 +246,7 @@ struct SolveWork {
     hipGraphExec_t exec = nullptr;
     int lanesA = 0;
     int chunkA = 0;
+    int mv_blocks = 0;  // grid of the fused SpMM + dots kernel (0: not used)
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -304,10 +305,13 @@ static const V* mat_val(const DCsr& A) {
 }
 
 // k sweeps of l1-Jacobi; (cur, oth) is the ping-pong pair, cur holds the
-// guess (ignored when xzero).  Returns the buffer holding the result.
+// guess (ignored when xzero).  Returns the buffer holding the result.  With
+// `out64` (FP32 sweeps only) the last sweep (one from a nonzero guess) writes
+// its result widened to FP64 there instead, and nullptr is returned; with
+// `outv` it writes it there (same type) and returns outv.
 template <typename V>
 static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const V* F, V* cur, V* oth, int s, bool xzero,
-                        int k, const int* stop, int chunk = 0) {
+                        int k, const int* stop, int chunk = 0, double* out64 = nullptr, V* outv = nullptr) {
     int done = 0;
     if (s <= 0) {
         if (xzero) fill<V>(c, cur, (long long)A.n * k, (V)0);
@@ -321,6 +325,17 @@ static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const 
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
         a.x = cur; a.y = oth; a.b = F; a.D = M; a.k = k; a.lanes = lanes; a.stop = stop; a.kchunk = chunk;
+        bool last = done + 1 == s;
+        if (last && out64) {
+            a.y64 = out64;
+            spmv<V, V>(c, SpOp::JACOBI, a);
+            return nullptr;
+        }
+        if (last && outv) {
+            a.y = outv;
+            spmv<V, V>(c, SpOp::JACOBI, a);
+            return outv;
+        }
         spmv<V, V>(c, SpOp::JACOBI, a);
         std::swap(cur, oth);
     }
@@ -351,7 +366,8 @@ static const V* lvl_pval(const Level& l) {
 // buffer (one of W[l].X / W[l].X2) or nullptr for zero.  Returns the buffer
 // holding the level's result.
 template <typename V>
-static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep_done = false) {
+static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep_done = false, double* out64 = nullptr,
+                     V* outv = nullptr) {
     Context* c = R.c;
     Prec* pr = R.pr;
     const bcmgx_params& p = pr->p;
@@ -418,7 +434,7 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
     if (xc) prolong_add<V>(c, A.n, k, lv.agg.get(), lvl_pval<V>(lv), xc, cur, R.stop);
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
     cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.postrelax_sweeps * coeff, false, k, R.stop,
-                           lv.chunkA);
+                           lv.chunkA, out64, outv);
     return cur;
 }
 
@@ -460,9 +476,10 @@ void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop
         if (p.precision == BCMGX_PRECISION_FP32) {
             CycleRun<float> R{c, this, &wf, k, stop};
             float* res;
+            // the last level-0 sweep writes u in FP64 (res == nullptr) when the cycle ends in one
             if (entry_ready) {
                 BCMGX_CHECK(x_zero && H.levels() >= 2 && p.prerelax_sweeps >= 1, BCMGX_ERR_STATE, "level-0 entry not applicable");
-                res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true);
+                res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, u);
             } else {
                 convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
                 float* xin = nullptr;
@@ -470,9 +487,9 @@ void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop
                     convert_stop<double, float>(c, u, wf[0].X.get(), nk, stop);
                     xin = wf[0].X.get();
                 }
-                res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin);
+                res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin, false, u);
             }
-            convert_stop<float, double>(c, res, u, nk, stop);
+            if (res) convert_stop<float, double>(c, res, u, nk, stop);
         } else {
             CycleRun<double> R{c, this, &wd, k, stop};
             double* xin = nullptr;
@@ -480,8 +497,9 @@ void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop
                 copy_d2d(c, wd[0].X.get(), u, sizeof(double) * nk);
                 xin = wd[0].X.get();
             }
-            double* res = gamg_cycle<double>(R, 0, r, xin);
-            convert_stop<double, double>(c, res, u, nk, stop);
+            // the last level-0 sweep writes u directly when the cycle ends in one
+            double* res = gamg_cycle<double>(R, 0, r, xin, false, nullptr, u);
+            if (res != u) convert_stop<double, double>(c, res, u, nk, stop);
         }
         break;
     }
@@ -547,8 +565,12 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
     a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
-    spmv<double, double>(c, SpOp::AX, a);
-    fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1);
+    if (s.mv_blocks > 0 && spmm_dots(c, a, s.r.get(), W1, s.fb.partials, s.mv_blocks, s.fb.all_done)) {
+        fcg_scalars3(c, s.fb, s.mv_blocks);
+    } else {
+        spmv<double, double>(c, SpOp::AX, a);
+        fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1);
+    }
     fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get(), s.entry);
     fcg_finish_iteration(c, s.fb);
 }
@@ -573,7 +595,8 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         s.fb.n = n;
         s.fb.k = k;
         s.fb.nblk = fcg_partial_blocks(n);
-        s.partials.reset(c, (size_t)s.fb.nblk * k * 3);
+        s.mv_blocks = spmm_dots_blocks(n, k);
+        s.partials.reset(c, (size_t)std::max(s.fb.nblk, s.mv_blocks) * k * 3);
         s.st.reset(c, k);
         s.flags.reset(c, 2);
         s.fb.partials = s.partials.get();

@@ -194,6 +194,11 @@ __global__ __launch_bounds__(WG) void k_scalars3(const double* __restrict__ part
     }
 }
 
+void fcg_scalars3(Context* c, const FcgBufs& f, int nblk) {
+    k_scalars3<<<f.k, WG, 0, c->stream>>>(f.partials, nblk, f.k, f.st, f.all_done);
+    BCMGX_LAUNCHED();
+}
+
 void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1) {
     int KP = kpad(f.k);
     k_dots<3><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, p0, w0, p0, w1, p0, f.partials, f.all_done);

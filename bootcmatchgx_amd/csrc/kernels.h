@@ -44,10 +44,20 @@ struct SpmvArgs {
     int lanes = 0;              // lanes per row for k == 1 (0 = heuristic, < 0 = CSR-stream kernel)
     int kchunk = 0;             // k > 1: columns per launch (0 = all k)
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
+    double* y64 = nullptr;      // JACOBI with V = float: write the result widened to FP64 here (y is scratch)
 };
 // T = matrix value type, V = vector type: (double,double), (float,float), (float,double)
 template <typename T, typename V>
 void spmv(Context* c, SpOp op, const SpmvArgs& a);
+
+// FCG: w = A p (FP64, k interleaved columns) fused with the per-column dot
+// products (r.p, w.p, wold.p) written as partials[(blk*k + c)*3 + m] of
+// `nblk` = spmm_dots_blocks(n, k) blocks.  Returns false (nothing launched)
+// when the shape does not fit the fused kernel (k odd or < 4, column chunks,
+// unaligned vectors); the caller then runs the SpMM and the dots apart.
+int spmm_dots_blocks(int n, int k);
+bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wold, double* partials, int nblk,
+               const int* all_done);
 
 // lanes-per-row heuristic from nnz/row (wave64 adaptation of CSR.cu:614-652)
 int default_lanes(long long nnz, int n);
@@ -95,6 +105,8 @@ void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const d
                      const FcgEntry& e = FcgEntry());
 // (alpha, beta, gamma) = (r.p0, w0.p0, w1.p0), then theta, delta, theta2 (FCG.cu:113-136)
 void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1);
+// the scalar step of fcg_dots3 alone, over `nblk` partials already written
+void fcg_scalars3(Context* c, const FcgBufs& f, int nblk);
 // double_merged_axpy x2 (FCG.cu:138-144) fused with the ||r||^2 partials
 void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r,
                 const FcgEntry& e = FcgEntry());

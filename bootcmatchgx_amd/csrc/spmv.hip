@@ -113,12 +113,13 @@ __global__ __launch_bounds__(WG) void k_spmmk(int n, int k, int c0, int kc, cons
 // columns of the x rows, U gathers in flight.  Each column's sum runs over the
 // row in order, as in k_spmmk, so the results are the same bits.  Needs
 // k % CPL == 0 and KP * CPL >= k; lanes past k / CPL idle.
-template <typename T, typename V, int KP, int CPL, int M, SpOp OP>
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, bool OUT64 = false>
 __global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
                                               const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
                                               const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
-                                              const int* __restrict__ stop) {
+                                              const int* __restrict__ stop, double* __restrict__ y64 = nullptr) {
     typedef V VV __attribute__((ext_vector_type(CPL)));
+    typedef double VD __attribute__((ext_vector_type(CPL)));
     if (stop && *stop) return;
     constexpr int RPB = WG / KP;
     constexpr int E = KP * M;
@@ -166,11 +167,147 @@ __global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restric
     }
     if (!active) return;
     long long o = (long long)r * k + cl * CPL;
+    if constexpr (OUT64) {  // JACOBI result widened to FP64 (the last sweep of an FP32 cycle)
+        VV res = ((-t + *(const VV*)(b + o)) / D[r]) + *(const VV*)(x + o);
+        *(VD*)(y64 + o) = __builtin_convertvector(res, VD);
+        return;
+    }
     VV* yo = (VV*)(y + o);
     if (OP == SpOp::AX) *yo = t;
     else if (OP == SpOp::RES) *yo = *(const VV*)(b + o) - t;
     else if (OP == SpOp::AXPBY) *yo = t + beta * *yo;
     else *yo = ((-t + *(const VV*)(b + o)) / D[r]) + *(const VV*)(x + o);
+}
+
+// FCG's w = A p fused with its triple dot product (alpha, beta, gamma) =
+// (r.p, w.p, w_old.p) per column (triple_innerproduct,
+// src/op/triple_inner_product.cu:7-101).  Same row kernel as k_spmmv (AX), on
+// a fixed persistent grid: the rows are cut into 8 contiguous segments, one
+// per XCD, and the blocks of an XCD stride through its segment.  Each block
+// reduces its rows in a fixed order and writes one partial per column, so
+// the sums depend on n only, never on scheduling.
+template <int KP, int CPL, int M>
+__global__ __launch_bounds__(WG) void k_spmm_dots(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
+                                                  const double* __restrict__ val, const double* __restrict__ p,
+                                                  double* __restrict__ w, const double* __restrict__ r,
+                                                  const double* __restrict__ wold, double* __restrict__ partials,
+                                                  const int* __restrict__ all_done) {
+    typedef double VV __attribute__((ext_vector_type(CPL)));
+    if (*all_done) return;
+    constexpr int RPB = WG / KP;
+    constexpr int E = KP * M;
+    constexpr int U = 16;
+    __shared__ int scol[WG * M];
+    __shared__ double sval[WG * M];
+    __shared__ double red[3 * CPL][WG];
+    int tid = threadIdx.x;
+    int cl = tid % KP;
+    int g = tid / KP;
+    bool active = cl * CPL < k;
+    // row groups of this XCD's segment, strided over its blocks
+    const int X = 8;
+    int xcd = blockIdx.x % X, local = blockIdx.x / X, nloc = gridDim.x / X;
+    int ngroups = (n + RPB - 1) / RPB;
+    int g0 = (int)((long long)ngroups * xcd / X), g1 = (int)((long long)ngroups * (xcd + 1) / X);
+    int* sc = scol + g * E;
+    double* sv = sval + g * E;
+    const double* pc = p + cl * CPL;
+    VV da = (VV)0.0, db = (VV)0.0, dc = (VV)0.0;
+    for (int grp = g0 + local; grp < g1; grp += nloc) {
+        int ri = grp * RPB + g;
+        bool live = ri < n;  // uniform over the lanes of a row
+        int s = live ? row[ri] : 0, e = live ? row[ri + 1] : 0;
+        VV t = (VV)0.0;
+        for (int base = s; base < e; base += E) {
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                int j = base + m * KP + cl;
+                sc[m * KP + cl] = j < e ? col[j] : 0;
+                sv[m * KP + cl] = j < e ? val[j] : 0.0;
+            }
+            wave_lds_sync();
+            int cnt = e - base < E ? e - base : E;
+            if (active) {
+                for (int q = 0; q < cnt; q += U) {
+                    VV xv[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        xv[u] = q + u < cnt ? *(const VV*)(pc + (long long)sc[q + u] * k) : (VV)0.0;
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (q + u < cnt) t += sv[q + u] * xv[u];
+                }
+            }
+            wave_lds_sync();
+        }
+        if (live && active) {
+            long long o = (long long)ri * k + cl * CPL;
+            *(VV*)(w + o) = t;
+            VV pv = *(const VV*)(p + o);
+            da += *(const VV*)(r + o) * pv;
+            db += t * pv;
+            dc += *(const VV*)(wold + o) * pv;
+        }
+    }
+    // fixed-order block reduction over the row groups, per column
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        red[3 * q + 0][tid] = da[q];
+        red[3 * q + 1][tid] = db[q];
+        red[3 * q + 2][tid] = dc[q];
+    }
+    __syncthreads();
+    for (int st = WG / 2; st >= KP; st >>= 1) {
+        if (tid < st) {
+#pragma unroll
+            for (int m = 0; m < 3 * CPL; m++) red[m][tid] += red[m][tid + st];
+        }
+        __syncthreads();
+    }
+    if (tid < KP && active) {
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            int c = tid * CPL + q;
+            double* pp = partials + ((long long)blockIdx.x * k + c) * 3;
+            pp[0] = red[3 * q + 0][tid];
+            pp[1] = red[3 * q + 1][tid];
+            pp[2] = red[3 * q + 2][tid];
+        }
+    }
+}
+
+int spmm_dots_blocks(int n, int k) {
+    if (k < 4 || k % 2 || k > 64) return 0;
+    int lanes = k / 2, KP = 2;
+    while (KP < lanes) KP <<= 1;
+    long long groups = ((long long)n + (WG / KP) - 1) / (WG / KP);
+    long long b = std::min<long long>(2048, std::max<long long>(8, (groups + 7) / 8 * 8));
+    return (int)(b / 8 * 8);
+}
+
+bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wold, double* partials, int nblk,
+               const int* all_done) {
+    constexpr int CPL = 2;
+    bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)r | (uintptr_t)wold) & 15) == 0;
+    if (nblk <= 0 || !aligned || a.k % CPL || a.k < 2 * CPL || (a.kchunk > 0 && a.kchunk < a.k)) return false;
+    int lanes = a.k / CPL;
+    int KP = 2;
+    while (KP < lanes) KP <<= 1;
+    const double* val = (const double*)a.val;
+    const double* p = (const double*)a.x;
+    double* w = (double*)a.y;
+#define BCMGX_D(KK, MM)                                                                                          \
+    case KK:                                                                                                     \
+        k_spmm_dots<KK, CPL, MM><<<nblk, WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, p, w, r, wold, partials, \
+                                                             all_done);                                          \
+        break;
+    switch (KP) {
+        BCMGX_D(2, 8) BCMGX_D(4, 8) BCMGX_D(8, 4) BCMGX_D(16, 2) BCMGX_D(32, 1)
+    default: return false;
+    }
+#undef BCMGX_D
+    BCMGX_LAUNCHED();
+    return true;
 }
 
 // CSR-stream SpMV for one vector: a workgroup owns RB = 256/TPR consecutive
@@ -310,15 +447,20 @@ static void launchk(Context* c, const SpmvArgs& a) {
     int kc = a.kchunk > 0 && a.kchunk < a.k ? a.kchunk : a.k;
     // whole rows of x as 16-byte vectors per lane: CPL = 16 / sizeof(V) columns
     constexpr int CPL = 16 / (int)sizeof(V);
-    bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b) & 15) == 0;
+    bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b | (uintptr_t)a.y64) & 15) == 0;
     if (kc == a.k && a.k % CPL == 0 && a.k >= 2 * CPL && aligned) {
         int lanes = a.k / CPL;
         int KP = 2;
         while (KP < lanes) KP <<= 1;
-#define BCMGX_V(KK, MM)                                                                                            \
-    case KK:                                                                                                       \
-        k_spmmv<T, V, KK, CPL, MM, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, x, \
-                                                                                     y, b, D, al, be, a.stop);     \
+        bool out64 = OP == SpOp::JACOBI && a.y64 != nullptr;
+#define BCMGX_V(KK, MM)                                                                                               \
+    case KK:                                                                                                          \
+        if (out64)                                                                                                    \
+            k_spmmv<T, V, KK, CPL, MM, OP, true><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(                       \
+                a.n, a.k, a.row, a.col, val, x, y, b, D, al, be, a.stop, a.y64);                                      \
+        else                                                                                                          \
+            k_spmmv<T, V, KK, CPL, MM, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, \
+                                                                                        x, y, b, D, al, be, a.stop);  \
         break;
         switch (KP) {
             BCMGX_V(2, 8) BCMGX_V(4, 8) BCMGX_V(8, 4) BCMGX_V(16, 2) BCMGX_V(32, 1)
@@ -349,6 +491,22 @@ static void launchk(Context* c, const SpmvArgs& a) {
 template <typename T, typename V>
 void spmv(Context* c, SpOp op, const SpmvArgs& a) {
     if (a.n <= 0) return;
+    if (a.y64) {
+        // JACOBI into y (V) and widened into y64: the vector SpMM writes y64
+        // directly; every other shape sweeps into y and converts
+        if (op != SpOp::JACOBI || !std::is_same<V, float>::value || !a.y)
+            throw Error(BCMGX_ERR_STATE, "spmv: FP64 output only for an FP32 Jacobi sweep");
+        constexpr int CPL = 16 / (int)sizeof(V);
+        int kc = a.kchunk > 0 && a.kchunk < a.k ? a.kchunk : a.k;
+        bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)a.b | (uintptr_t)a.y64) & 15) == 0;
+        if (!(a.k > 1 && kc == a.k && a.k % CPL == 0 && a.k >= 2 * CPL && aligned)) {
+            SpmvArgs b = a;
+            b.y64 = nullptr;
+            spmv<T, V>(c, op, b);
+            convert_stop<V, double>(c, (const V*)a.y, a.y64, (long long)a.n * a.k, a.stop);
+            return;
+        }
+    }
     // operands the kernels dereference: a null one is a library bug, reported instead of launched
     if (!a.row || (a.nnz > 0 && (!a.col || !a.val || !a.x)) || !a.y || a.k < 1 ||
         ((op == SpOp::RES || op == SpOp::JACOBI) && !a.b) || (op == SpOp::JACOBI && (!a.D || !a.x)))
