@@ -595,7 +595,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         s.fb.n = n;
         s.fb.k = k;
         s.fb.nblk = fcg_partial_blocks(n);
-        s.mv_blocks = spmm_dots_blocks(n, k);
+        s.mv_blocks = spmm_dots_blocks(c, n, k);
         s.partials.reset(c, (size_t)std::max(s.fb.nblk, s.mv_blocks) * k * 3);
         s.st.reset(c, k);
         s.flags.reset(c, 2);

@@ -55,7 +55,7 @@ void spmv(Context* c, SpOp op, const SpmvArgs& a);
 // `nblk` = spmm_dots_blocks(n, k) blocks.  Returns false (nothing launched)
 // when the shape does not fit the fused kernel (k odd or < 4, column chunks,
 // unaligned vectors); the caller then runs the SpMM and the dots apart.
-int spmm_dots_blocks(int n, int k);
+int spmm_dots_blocks(Context* c, int n, int k);
 bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wold, double* partials, int nblk,
                const int* all_done);
 

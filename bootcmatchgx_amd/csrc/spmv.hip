@@ -179,135 +179,253 @@ __global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restric
     else *yo = ((-t + *(const VV*)(b + o)) / D[r]) + *(const VV*)(x + o);
 }
 
-// FCG's w = A p fused with its triple dot product (alpha, beta, gamma) =
-// (r.p, w.p, w_old.p) per column (triple_innerproduct,
-// src/op/triple_inner_product.cu:7-101).  Same row kernel as k_spmmv (AX), on
-// a fixed persistent grid: the rows are cut into 8 contiguous segments, one
-// per XCD, and the blocks of an XCD stride through its segment.  Each block
-// reduces its rows in a fixed order and writes one partial per column, so
-// the sums depend on n only, never on scheduling.
-template <int KP, int CPL, int M>
-__global__ __launch_bounds__(WG) void k_spmm_dots(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
-                                                  const double* __restrict__ val, const double* __restrict__ p,
-                                                  double* __restrict__ w, const double* __restrict__ r,
-                                                  const double* __restrict__ wold, double* __restrict__ partials,
-                                                  const int* __restrict__ all_done) {
-    typedef double VV __attribute__((ext_vector_type(CPL)));
-    if (*all_done) return;
+// Operands of the pipelined SpMM's optional epilogues.
+struct SpmmExtra {
+    double* y64 = nullptr;         // MODE 1: JACOBI result widened to FP64
+    const double* r = nullptr;     // MODE 2: FCG residual
+    const double* wold = nullptr;  // MODE 2: previous A-direction
+    double* partials = nullptr;    // MODE 2: per-block dot partials
+};
+
+// Software-pipelined persistent k-vector SpMM, CPL consecutive columns per
+// lane (16-byte gathers), KP lanes per row.  The rows are cut into 8
+// contiguous segments, one per XCD (hardware dispatch puts block b on XCD
+// b % 8), and the blocks of an XCD stride through its segment.  Each KP-lane
+// group walks its own sequence of rows with a three-stage pipeline: the row
+// bounds of row i+2 and the first M*KP (col, val) entries of row i+1 are in
+// flight while row i's x gathers (16 per lane) are.  Every column's sum runs
+// over the row in order, so results are bit-identical to the one-vector
+// kernels' sequential order.
+//   MODE 0: y = op(A x);  MODE 1: JACOBI written widened to FP64 (ex.y64);
+//   MODE 2 (FP64, AX): w = A p and FCG's triple dot product (r.p, w.p, wold.p)
+//          per column (triple_innerproduct, src/op/triple_inner_product.cu:7-101),
+//          reduced per block in a fixed order into ex.partials[(blk*k + c)*3 + m].
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE>
+__global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
+                                              const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
+                                              const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
+                                              const int* __restrict__ stop, SpmmExtra ex) {
+    typedef V VV __attribute__((ext_vector_type(CPL)));
+    typedef double VD __attribute__((ext_vector_type(CPL)));
+    if (stop && *stop) return;
     constexpr int RPB = WG / KP;
     constexpr int E = KP * M;
     constexpr int U = 16;
     __shared__ int scol[WG * M];
-    __shared__ double sval[WG * M];
-    __shared__ double red[3 * CPL][WG];
+    __shared__ T sval[WG * M];
     int tid = threadIdx.x;
     int cl = tid % KP;
     int g = tid / KP;
     bool active = cl * CPL < k;
-    // row groups of this XCD's segment, strided over its blocks
     const int X = 8;
     int xcd = blockIdx.x % X, local = blockIdx.x / X, nloc = gridDim.x / X;
     int ngroups = (n + RPB - 1) / RPB;
     int g0 = (int)((long long)ngroups * xcd / X), g1 = (int)((long long)ngroups * (xcd + 1) / X);
     int* sc = scol + g * E;
-    double* sv = sval + g * E;
-    const double* pc = p + cl * CPL;
-    VV da = (VV)0.0, db = (VV)0.0, dc = (VV)0.0;
-    for (int grp = g0 + local; grp < g1; grp += nloc) {
+    T* sv = sval + g * E;
+    const V* xc = x + cl * CPL;
+    VD da = (VD)0.0, db = (VD)0.0, dc = (VD)0.0;
+    auto bounds = [&](int grp, int& rs, int& re) {
         int ri = grp * RPB + g;
-        bool live = ri < n;  // uniform over the lanes of a row
-        int s = live ? row[ri] : 0, e = live ? row[ri + 1] : 0;
-        VV t = (VV)0.0;
-        for (int base = s; base < e; base += E) {
+        if (grp < g1 && ri < n) {
+            rs = row[ri];
+            re = row[ri + 1];
+        } else {
+            rs = 0;
+            re = 0;
+        }
+    };
+    auto chunk = [&](int base, int re, int* pc, T* pv) {
 #pragma unroll
-            for (int m = 0; m < M; m++) {
-                int j = base + m * KP + cl;
-                sc[m * KP + cl] = j < e ? col[j] : 0;
-                sv[m * KP + cl] = j < e ? val[j] : 0.0;
-            }
-            wave_lds_sync();
-            int cnt = e - base < E ? e - base : E;
-            if (active) {
-                for (int q = 0; q < cnt; q += U) {
-                    VV xv[U];
+        for (int m = 0; m < M; m++) {
+            int j = base + m * KP + cl;
+            pc[m] = j < re ? col[j] : 0;
+            pv[m] = j < re ? val[j] : (T)0;
+        }
+    };
+    auto gather = [&](int cnt, VV& t) {
+        if (!active) return;
+        for (int q = 0; q < cnt; q += U) {
+            VV xv[U];
 #pragma unroll
-                    for (int u = 0; u < U; u++)
-                        xv[u] = q + u < cnt ? *(const VV*)(pc + (long long)sc[q + u] * k) : (VV)0.0;
+            for (int u = 0; u < U; u++) xv[u] = q + u < cnt ? *(const VV*)(xc + (long long)sc[q + u] * k) : (VV)(V)0;
 #pragma unroll
-                    for (int u = 0; u < U; u++)
-                        if (q + u < cnt) t += sv[q + u] * xv[u];
+            for (int u = 0; u < U; u++) {
+                if (q + u < cnt) {
+                    V a = (V)sv[q + u];
+                    if (OP == SpOp::AXPBY) t += (alpha * a) * xv[u];
+                    else t += a * xv[u];
                 }
             }
+        }
+    };
+    int grp = g0 + local;  // uniform over the block
+    int s0, e0, s1, e1;
+    bounds(grp, s0, e0);
+    bounds(grp + nloc, s1, e1);
+    int pc[M];
+    T pv[M];
+    chunk(s0, e0, pc, pv);
+    while (grp < g1) {
+        int s2, e2;
+        bounds(grp + 2 * nloc, s2, e2);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            sc[m * KP + cl] = pc[m];
+            sv[m * KP + cl] = pv[m];
+        }
+        wave_lds_sync();
+        int pc1[M];
+        T pv1[M];
+        chunk(s1, e1, pc1, pv1);  // next row's first entries, in flight during the gathers
+        VV t = (VV)(V)0;
+        gather(e0 - s0 < E ? e0 - s0 : E, t);
+        wave_lds_sync();
+        for (int base = s0 + E; base < e0; base += E) {  // rows longer than E: the rest unpipelined
+            int qc[M];
+            T qv[M];
+            chunk(base, e0, qc, qv);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                sc[m * KP + cl] = qc[m];
+                sv[m * KP + cl] = qv[m];
+            }
+            wave_lds_sync();
+            gather(e0 - base < E ? e0 - base : E, t);
             wave_lds_sync();
         }
-        if (live && active) {
+        int ri = grp * RPB + g;
+        if (ri < n && active) {
             long long o = (long long)ri * k + cl * CPL;
-            *(VV*)(w + o) = t;
-            VV pv = *(const VV*)(p + o);
-            da += *(const VV*)(r + o) * pv;
-            db += t * pv;
-            dc += *(const VV*)(wold + o) * pv;
+            if constexpr (MODE == 1) {
+                VV res = ((-t + *(const VV*)(b + o)) / D[ri]) + *(const VV*)(x + o);
+                *(VD*)(ex.y64 + o) = __builtin_convertvector(res, VD);
+            } else if constexpr (MODE == 2) {
+                *(VV*)(y + o) = t;
+                VD pw = *(const VD*)(x + o);
+                da += *(const VD*)(ex.r + o) * pw;
+                db += t * pw;
+                dc += *(const VD*)(ex.wold + o) * pw;
+            } else {
+                VV* yo = (VV*)(y + o);
+                if (OP == SpOp::AX) *yo = t;
+                else if (OP == SpOp::RES) *yo = *(const VV*)(b + o) - t;
+                else if (OP == SpOp::AXPBY) *yo = t + beta * *yo;
+                else *yo = ((-t + *(const VV*)(b + o)) / D[ri]) + *(const VV*)(x + o);
+            }
+        }
+        grp += nloc;
+        s0 = s1; e0 = e1;
+        s1 = s2; e1 = e2;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            pc[m] = pc1[m];
+            pv[m] = pv1[m];
         }
     }
-    // fixed-order block reduction over the row groups, per column
-#pragma unroll
-    for (int q = 0; q < CPL; q++) {
-        red[3 * q + 0][tid] = da[q];
-        red[3 * q + 1][tid] = db[q];
-        red[3 * q + 2][tid] = dc[q];
-    }
-    __syncthreads();
-    for (int st = WG / 2; st >= KP; st >>= 1) {
-        if (tid < st) {
-#pragma unroll
-            for (int m = 0; m < 3 * CPL; m++) red[m][tid] += red[m][tid + st];
-        }
-        __syncthreads();
-    }
-    if (tid < KP && active) {
+    if constexpr (MODE == 2) {
+        // fixed-order block reduction over the row groups, per column
+        __shared__ double red[3 * CPL][WG];
 #pragma unroll
         for (int q = 0; q < CPL; q++) {
-            int c = tid * CPL + q;
-            double* pp = partials + ((long long)blockIdx.x * k + c) * 3;
-            pp[0] = red[3 * q + 0][tid];
-            pp[1] = red[3 * q + 1][tid];
-            pp[2] = red[3 * q + 2][tid];
+            red[3 * q + 0][tid] = da[q];
+            red[3 * q + 1][tid] = db[q];
+            red[3 * q + 2][tid] = dc[q];
+        }
+        __syncthreads();
+        for (int st = WG / 2; st >= KP; st >>= 1) {
+            if (tid < st) {
+#pragma unroll
+                for (int m = 0; m < 3 * CPL; m++) red[m][tid] += red[m][tid + st];
+            }
+            __syncthreads();
+        }
+        if (tid < KP && active) {
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                double* pp = ex.partials + ((long long)blockIdx.x * k + tid * CPL + q) * 3;
+                pp[0] = red[3 * q + 0][tid];
+                pp[1] = red[3 * q + 1][tid];
+                pp[2] = red[3 * q + 2][tid];
+            }
         }
     }
 }
 
-int spmm_dots_blocks(int n, int k) {
-    if (k < 4 || k % 2 || k > 64) return 0;
-    int lanes = k / 2, KP = 2;
+// resident blocks of one pipelined-SpMM instantiation on the whole device, a multiple of 8
+template <typename K>
+static int persistent_grid(Context* c, K kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, WG, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    int g = c->num_cus * per_cu;
+    return std::max(8, g / 8 * 8);
+}
+
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE>
+static void launch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int grid) {
+    auto kern = k_spmmp<T, V, KP, CPL, M, OP, MODE>;
+    if (grid <= 0) {
+        static int cached = 0;  // per instantiation; identical devices assumed within a process
+        if (!cached) cached = persistent_grid(c, kern);
+        grid = cached;
+    }
+    kern<<<grid, WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, (const T*)a.val, (const V*)a.x, (V*)a.y, (const V*)a.b,
+                                     (const V*)a.D, (V)a.alpha, (V)a.beta, a.stop, ex);
+    BCMGX_LAUNCHED();
+}
+
+// lanes per row of the pipelined SpMM for k columns of CPL per lane, or 0 when it does not apply
+static int spmmp_lanes(int k, int cpl) {
+    if (k % cpl || k < 2 * cpl || k > 64) return 0;
+    int lanes = k / cpl, KP = 2;
     while (KP < lanes) KP <<= 1;
-    long long groups = ((long long)n + (WG / KP) - 1) / (WG / KP);
-    long long b = std::min<long long>(2048, std::max<long long>(8, (groups + 7) / 8 * 8));
-    return (int)(b / 8 * 8);
+    return KP;
+}
+
+template <typename T, typename V, SpOp OP, int MODE>
+static bool dispatch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int grid) {
+    constexpr int CPL = 16 / (int)sizeof(V);
+    switch (spmmp_lanes(a.k, CPL)) {
+    case 2: launch_spmmp<T, V, 2, CPL, 8, OP, MODE>(c, a, ex, grid); return true;
+    case 4: launch_spmmp<T, V, 4, CPL, 8, OP, MODE>(c, a, ex, grid); return true;
+    case 8: launch_spmmp<T, V, 8, CPL, 4, OP, MODE>(c, a, ex, grid); return true;
+    case 16: launch_spmmp<T, V, 16, CPL, 2, OP, MODE>(c, a, ex, grid); return true;
+    case 32: launch_spmmp<T, V, 32, CPL, 1, OP, MODE>(c, a, ex, grid); return true;
+    default: return false;
+    }
+}
+
+template <typename T, typename V, SpOp OP, int MODE>
+static int spmmp_grid(Context* c, int k) {
+    constexpr int CPL = 16 / (int)sizeof(V);
+    switch (spmmp_lanes(k, CPL)) {
+    case 2: return persistent_grid(c, k_spmmp<T, V, 2, CPL, 8, OP, MODE>);
+    case 4: return persistent_grid(c, k_spmmp<T, V, 4, CPL, 8, OP, MODE>);
+    case 8: return persistent_grid(c, k_spmmp<T, V, 8, CPL, 4, OP, MODE>);
+    case 16: return persistent_grid(c, k_spmmp<T, V, 16, CPL, 2, OP, MODE>);
+    case 32: return persistent_grid(c, k_spmmp<T, V, 32, CPL, 1, OP, MODE>);
+    default: return 0;
+    }
+}
+
+// grid of the fused FCG kernel for k columns: the device's resident block
+// count, fixed for a device, so the partial count never changes between calls
+int spmm_dots_blocks(Context* c, int n, int k) {
+    (void)n;
+    return spmmp_grid<double, double, SpOp::AX, 2>(c, k);
 }
 
 bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wold, double* partials, int nblk,
                const int* all_done) {
-    constexpr int CPL = 2;
     bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)r | (uintptr_t)wold) & 15) == 0;
-    if (nblk <= 0 || !aligned || a.k % CPL || a.k < 2 * CPL || (a.kchunk > 0 && a.kchunk < a.k)) return false;
-    int lanes = a.k / CPL;
-    int KP = 2;
-    while (KP < lanes) KP <<= 1;
-    const double* val = (const double*)a.val;
-    const double* p = (const double*)a.x;
-    double* w = (double*)a.y;
-#define BCMGX_D(KK, MM)                                                                                          \
-    case KK:                                                                                                     \
-        k_spmm_dots<KK, CPL, MM><<<nblk, WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, p, w, r, wold, partials, \
-                                                             all_done);                                          \
-        break;
-    switch (KP) {
-        BCMGX_D(2, 8) BCMGX_D(4, 8) BCMGX_D(8, 4) BCMGX_D(16, 2) BCMGX_D(32, 1)
-    default: return false;
-    }
-#undef BCMGX_D
-    BCMGX_LAUNCHED();
-    return true;
+    if (nblk <= 0 || nblk % 8 || !aligned || !spmmp_lanes(a.k, 2) || (a.kchunk > 0 && a.kchunk < a.k)) return false;
+    SpmmExtra ex;
+    ex.r = r;
+    ex.wold = wold;
+    ex.partials = partials;
+    SpmvArgs b = a;
+    b.stop = all_done;
+    return dispatch_spmmp<double, double, SpOp::AX, 2>(c, b, ex, nblk);
 }
 
 // CSR-stream SpMV for one vector: a workgroup owns RB = 256/TPR consecutive
@@ -448,26 +566,14 @@ static void launchk(Context* c, const SpmvArgs& a) {
     // whole rows of x as 16-byte vectors per lane: CPL = 16 / sizeof(V) columns
     constexpr int CPL = 16 / (int)sizeof(V);
     bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b | (uintptr_t)a.y64) & 15) == 0;
-    if (kc == a.k && a.k % CPL == 0 && a.k >= 2 * CPL && aligned) {
-        int lanes = a.k / CPL;
-        int KP = 2;
-        while (KP < lanes) KP <<= 1;
-        bool out64 = OP == SpOp::JACOBI && a.y64 != nullptr;
-#define BCMGX_V(KK, MM)                                                                                               \
-    case KK:                                                                                                          \
-        if (out64)                                                                                                    \
-            k_spmmv<T, V, KK, CPL, MM, OP, true><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(                       \
-                a.n, a.k, a.row, a.col, val, x, y, b, D, al, be, a.stop, a.y64);                                      \
-        else                                                                                                          \
-            k_spmmv<T, V, KK, CPL, MM, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, \
-                                                                                        x, y, b, D, al, be, a.stop);  \
-        break;
-        switch (KP) {
-            BCMGX_V(2, 8) BCMGX_V(4, 8) BCMGX_V(8, 4) BCMGX_V(16, 2) BCMGX_V(32, 1)
-        default: throw Error(BCMGX_ERR_ARG, "at most 64 right-hand sides");
+    if (kc == a.k && aligned && spmmp_lanes(a.k, CPL)) {
+        SpmmExtra ex;
+        if (OP == SpOp::JACOBI && a.y64) {
+            ex.y64 = a.y64;
+            dispatch_spmmp<T, V, OP, 1>(c, a, ex, 0);
+        } else {
+            dispatch_spmmp<T, V, OP, 0>(c, a, ex, 0);
         }
-#undef BCMGX_V
-        BCMGX_LAUNCHED();
         return;
     }
     for (int c0 = 0; c0 < a.k; c0 += kc) {
