@@ -107,10 +107,16 @@ int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
     BCMGX_HIP(hipEventCreate(&e1));
     int best = k;
     float bestms = 1e30f;
-    for (int ch = k; ch >= 4; ch = (ch + 1) / 2) {
+    // full width through the pipelined (k), one-pass (-1) or pipelined kernel with
+    // streamed matrix loads (-2), then narrower column chunks
+    std::vector<int> cands = {k, -1, -2};
+    for (int ch = (k + 1) / 2; ch >= 4; ch = (ch + 1) / 2) {
+        cands.push_back(ch);
+        if (ch == 4) break;
+    }
+    for (int ch : cands) {
         float ms = fp32 ? time_spmm<float>(c, A, A.valf.get(), k, ch, e0, e1) : time_spmm<double>(c, A, A.val.get(), k, ch, e0, e1);
         if (ms < bestms) { bestms = ms; best = ch; }
-        if (ch == 4) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

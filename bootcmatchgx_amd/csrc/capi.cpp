@@ -595,7 +595,7 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
 
 int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
                double beta, int op, int kchunk, int mem) {
-    if (!ctx || !A || !x || !y || op < 0 || op > 2 || k < 1 || k > 64 || kchunk < 0) return BCMGX_ERR_ARG;
+    if (!ctx || !A || !x || !y || op < 0 || op > 2 || k < 1 || k > 64 || kchunk < -2) return BCMGX_ERR_ARG;
     API_BEGIN
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
@@ -630,7 +630,7 @@ int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int
         convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
     }
     if (k == 1) a.lanes = lanes != 0 ? lanes : tune_lanes(c, M, precision == 1);
-    else a.kchunk = lanes > 0 ? lanes : 0;
+    else a.kchunk = lanes >= -2 ? lanes : 0;
     DBuf<double> xd, yd;
     DBuf<float> xf, yf;
     if (precision == 1) {

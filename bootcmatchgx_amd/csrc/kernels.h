@@ -42,7 +42,8 @@ struct SpmvArgs {
     double alpha = 1.0, beta = 0.0;
     int k = 1;                  // number of vector columns
     int lanes = 0;              // lanes per row for k == 1 (0 = heuristic, < 0 = CSR-stream kernel)
-    int kchunk = 0;             // k > 1: columns per launch (0 = all k)
+    int kchunk = 0;             // k > 1: columns per launch (0 = all k, pipelined kernel; -1 = all k, one-pass
+                                // kernel; -2 = all k, pipelined kernel with streamed matrix loads)
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
     double* y64 = nullptr;      // JACOBI with V = float: write the result widened to FP64 here (y is scratch)
 };

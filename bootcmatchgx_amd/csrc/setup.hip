@@ -573,6 +573,8 @@ __global__ void k_count_products(int n, const int* __restrict__ arow, const int*
     cnt[i] = s;
 }
 
+// bins: 0..2 rows of <= 16 / 32 / 64 products (W-lane sorts), 3 medium, 4 large
+constexpr int NBINS = 5;
 // bin rows by product count; one atomic per wave and bin (order inside a bin is irrelevant)
 __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ arow, const long long* __restrict__ cnt,
                                                  int* __restrict__ bins, int* __restrict__ bincount, int cap) {
@@ -582,10 +584,11 @@ __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ 
     if (i < n) {
         long long p = cnt[i];
         int alen = arow[i + 1] - arow[i];
-        b = (p <= SMALL_P && alen <= SMALL_P) ? 0 : ((p <= MED_P && alen <= MED_P) ? 1 : 2);
+        long long m = p > alen ? p : alen;
+        b = m <= 16 ? 0 : m <= 32 ? 1 : m <= SMALL_P ? 2 : (m <= MED_P ? 3 : 4);
     }
     unsigned long long below = (1ull << lane) - 1ull;
-    for (int bb = 0; bb < 3; bb++) {
+    for (int bb = 0; bb < NBINS; bb++) {
         unsigned long long m = __ballot(b == bb);
         if (!m) continue;
         int leader = __ffsll((long long)m) - 1;
@@ -596,21 +599,25 @@ __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ 
     }
 }
 
-__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
-    return (unsigned long long)__shfl_xor((long long)v, m, 64);
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m, int w) {
+    return (unsigned long long)__shfl_xor((long long)v, m, w);
 }
 
-// one wave64 per row, products <= 64
+// W lanes per row (W = 16, 32 or 64), products <= W: 64 / W rows per wave
+template <int W>
 __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
                                                      const int* __restrict__ acol, const double* __restrict__ aval,
                                                      const int* __restrict__ brow, const int* __restrict__ bcol,
                                                      const double* __restrict__ bval, const long long* __restrict__ poff,
                                                      int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq) {
-    __shared__ int pre_s[4][SMALL_P + 1];
-    __shared__ double v_s[4][SMALL_P];
-    int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int w = blockIdx.x * 4 + wid;
-    bool active = w < nrows;  // inactive waves still take part in the barriers
+    constexpr int G = WG / W;  // rows per block
+    __shared__ int pre_s[G][W + 1];
+    __shared__ double v_s[G][W];
+    int gid = threadIdx.x / W, lane = threadIdx.x % W;
+    int wl = threadIdx.x & 63;             // lane in the wave
+    int gshift = wl - lane;                // first wave lane of this row's group
+    int w = blockIdx.x * G + gid;
+    bool active = w < nrows;  // inactive groups still take part in the barriers
     int i = active ? rows[w] : 0;
     int s = active ? arow[i] : 0, alen = active ? arow[i + 1] - s : 0;
     int k = -1, cntl = 0;
@@ -618,16 +625,16 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
         k = acol[s + lane];
         cntl = brow[k + 1] - brow[k];
     }
-    // inclusive scan of B-row lengths across the wave
+    // inclusive scan of B-row lengths across the group
     int x = cntl;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int y = __shfl_up(x, d, 64);
+    for (int d = 1; d < W; d <<= 1) {
+        int y = __shfl_up(x, d, W);
         if (lane >= d) x += y;
     }
-    int total = __shfl(x, 63, 64);
-    pre_s[wid][lane + 1] = x;
-    if (lane == 0) pre_s[wid][0] = 0;
+    int total = __shfl(x, W - 1, W);
+    pre_s[gid][lane + 1] = x;
+    if (lane == 0) pre_s[gid][0] = 0;
     __syncthreads();
     unsigned long long key = ~0ull;
     double v = 0.0;
@@ -636,23 +643,23 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
         int lo = 0, hi = alen;  // pre[0] = 0 <= lane
         while (hi - lo > 1) {
             int mid = (lo + hi) >> 1;
-            if (pre_s[wid][mid] <= lane) lo = mid;
+            if (pre_s[gid][mid] <= lane) lo = mid;
             else hi = mid;
         }
         int a = lo;
         int ka = acol[s + a];
-        int jb = brow[ka] + (lane - pre_s[wid][a]);
+        int jb = brow[ka] + (lane - pre_s[gid][a]);
         int cc = bcol[jb];
         v = aval[s + a] * bval[jb];
         key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)lane;
     }
-    // bitonic sort of 64 (key, v) pairs across the wave
+    // bitonic sort of W (key, v) pairs across the group
 #pragma unroll
-    for (int kk = 2; kk <= 64; kk <<= 1) {
+    for (int kk = 2; kk <= W; kk <<= 1) {
 #pragma unroll
         for (int j = kk >> 1; j > 0; j >>= 1) {
-            unsigned long long pk = shfl_xor_u64(key, j);
-            double pv = __shfl_xor(v, j, 64);
+            unsigned long long pk = shfl_xor_u64(key, j, W);
+            double pv = __shfl_xor(v, j, W);
             bool up = (lane & kk) == 0;
             bool lower = (lane & j) == 0;
             bool take = (lower == up) ? (pk < key) : (pk > key);
@@ -662,17 +669,18 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
             }
         }
     }
-    v_s[wid][lane] = v;
+    v_s[gid][lane] = v;
     int colv = (int)(key >> 32);
-    int prevc = __shfl_up(colv, 1, 64);
+    int prevc = __shfl_up(colv, 1, W);
     bool head = lane < total && (lane == 0 || prevc != colv);
     unsigned long long hm = __ballot(head);
+    hm = (hm >> gshift) & (W == 64 ? ~0ull : ((1ull << W) - 1ull));  // this group's lanes
     __syncthreads();
     if (head && active) {
         double acc = v;
         // sum the run left to right
         for (int t = lane + 1; t < total; t++) {
-            if (!((hm >> t) & 1ull)) acc += v_s[wid][t];
+            if (!((hm >> t) & 1ull)) acc += v_s[gid][t];
             else break;
         }
         int u = __popcll(hm & ((1ull << lane) - 1ull));
@@ -855,34 +863,40 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     DBuf<int> tcol(c, tsz);
     DBuf<double> tval(c, tsz);
     DBuf<int> uniq(c, (size_t)n + 1);
-    DBuf<int> bins(c, (size_t)3 * n), bincount(c, 3);
-    fill<int>(c, bincount.get(), 3, 0);
+    DBuf<int> bins(c, (size_t)NBINS * n), bincount(c, NBINS);
+    fill<int>(c, bincount.get(), NBINS, 0);
     k_bin_rows<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), cnt.get(), bins.get(), bincount.get(), n);
     BCMGX_LAUNCHED();
-    int bc[3];
-    c->d2h(bc, bincount.get(), 3);
-    if (bc[0] > 0) {
-        k_spgemm_small<<<grid_for(bc[0], 4), WG, 0, c->stream>>>(bc[0], bins.get(), A.row.get(), A.col.get(), A.val.get(),
-                                                                  B.row.get(), B.col.get(), B.val.get(), poff.get(),
-                                                                  tcol.get(), tval.get(), uniq.get());
+    int bc[NBINS];
+    c->d2h(bc, bincount.get(), NBINS);
+#define BCMGX_SMALL(BIN, W)                                                                                                 \
+    if (bc[BIN] > 0) {                                                                                                      \
+        k_spgemm_small<W><<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(                                                 \
+            bc[BIN], bins.get() + (size_t)(BIN) * n, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(),      \
+            B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get());                                                   \
+        BCMGX_LAUNCHED();                                                                                                   \
+    }
+    BCMGX_SMALL(0, 16)
+    BCMGX_SMALL(1, 32)
+    BCMGX_SMALL(2, 64)
+#undef BCMGX_SMALL
+    if (bc[3] > 0) {
+        k_spgemm_med<<<bc[3], WG, 0, c->stream>>>(bc[3], bins.get() + 3 * (size_t)n, A.row.get(), A.col.get(), A.val.get(),
+                                                  B.row.get(), B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(),
+                                                  uniq.get());
         BCMGX_LAUNCHED();
     }
-    if (bc[1] > 0) {
-        k_spgemm_med<<<bc[1], WG, 0, c->stream>>>(bc[1], bins.get() + n, A.row.get(), A.col.get(), A.val.get(), B.row.get(),
-                                                  B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get());
-        BCMGX_LAUNCHED();
-    }
-    if (bc[2] > 0) {
-        int nl = bc[2];
+    if (bc[4] > 0) {
+        int nl = bc[4];
         DBuf<long long> psz(c, nl), asz(c, nl), soff(c, (size_t)nl + 1), aoff(c, (size_t)nl + 1);
-        k_large_sizes<<<grid_for(nl, WG), WG, 0, c->stream>>>(nl, bins.get() + 2 * (size_t)n, cnt.get(), A.row.get(), psz.get(), asz.get());
+        k_large_sizes<<<grid_for(nl, WG), WG, 0, c->stream>>>(nl, bins.get() + 4 * (size_t)n, cnt.get(), A.row.get(), psz.get(), asz.get());
         BCMGX_LAUNCHED();
         long long ptot = scan_exclusive(c, psz.get(), soff.get(), nl);
         long long atot = scan_exclusive(c, asz.get(), aoff.get(), nl);
         DBuf<unsigned long long> gk(c, (size_t)ptot);
         DBuf<double> gv(c, (size_t)ptot);
         DBuf<int> gp(c, (size_t)atot);
-        k_spgemm_large<<<nl, WG, 0, c->stream>>>(nl, bins.get() + 2 * (size_t)n, A.row.get(), A.col.get(), A.val.get(),
+        k_spgemm_large<<<nl, WG, 0, c->stream>>>(nl, bins.get() + 4 * (size_t)n, A.row.get(), A.col.get(), A.val.get(),
                                                  B.row.get(), B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(),
                                                  uniq.get(), gk.get(), gv.get(), gp.get(), soff.get(), aoff.get());
         BCMGX_LAUNCHED();

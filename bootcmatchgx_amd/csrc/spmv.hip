@@ -200,7 +200,7 @@ struct SpmmExtra {
 //   MODE 2 (FP64, AX): w = A p and FCG's triple dot product (r.p, w.p, wold.p)
 //          per column (triple_innerproduct, src/op/triple_inner_product.cu:7-101),
 //          reduced per block in a fixed order into ex.partials[(blk*k + c)*3 + m].
-template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE>
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT = false>
 __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
                                               const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
                                               const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
@@ -239,8 +239,13 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
 #pragma unroll
         for (int m = 0; m < M; m++) {
             int j = base + m * KP + cl;
-            pc[m] = j < re ? col[j] : 0;
-            pv[m] = j < re ? val[j] : (T)0;
+            if constexpr (NT) {  // matrix entries streamed past the caches, which keep x
+                pc[m] = j < re ? __builtin_nontemporal_load(col + j) : 0;
+                pv[m] = j < re ? __builtin_nontemporal_load(val + j) : (T)0;
+            } else {
+                pc[m] = j < re ? col[j] : 0;
+                pv[m] = j < re ? val[j] : (T)0;
+            }
         }
     };
     auto gather = [&](int cnt, VV& t) {
@@ -361,9 +366,9 @@ static int persistent_grid(Context* c, K kernel) {
     return std::max(8, g / 8 * 8);
 }
 
-template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE>
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT>
 static void launch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int grid) {
-    auto kern = k_spmmp<T, V, KP, CPL, M, OP, MODE>;
+    auto kern = k_spmmp<T, V, KP, CPL, M, OP, MODE, NT>;
     if (grid <= 0) {
         static int cached = 0;  // per instantiation; identical devices assumed within a process
         if (!cached) cached = persistent_grid(c, kern);
@@ -382,15 +387,15 @@ static int spmmp_lanes(int k, int cpl) {
     return KP;
 }
 
-template <typename T, typename V, SpOp OP, int MODE>
+template <typename T, typename V, SpOp OP, int MODE, bool NT = false>
 static bool dispatch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int grid) {
     constexpr int CPL = 16 / (int)sizeof(V);
     switch (spmmp_lanes(a.k, CPL)) {
-    case 2: launch_spmmp<T, V, 2, CPL, 8, OP, MODE>(c, a, ex, grid); return true;
-    case 4: launch_spmmp<T, V, 4, CPL, 8, OP, MODE>(c, a, ex, grid); return true;
-    case 8: launch_spmmp<T, V, 8, CPL, 4, OP, MODE>(c, a, ex, grid); return true;
-    case 16: launch_spmmp<T, V, 16, CPL, 2, OP, MODE>(c, a, ex, grid); return true;
-    case 32: launch_spmmp<T, V, 32, CPL, 1, OP, MODE>(c, a, ex, grid); return true;
+    case 2: launch_spmmp<T, V, 2, CPL, 8, OP, MODE, NT>(c, a, ex, grid); return true;
+    case 4: launch_spmmp<T, V, 4, CPL, 8, OP, MODE, NT>(c, a, ex, grid); return true;
+    case 8: launch_spmmp<T, V, 8, CPL, 4, OP, MODE, NT>(c, a, ex, grid); return true;
+    case 16: launch_spmmp<T, V, 16, CPL, 2, OP, MODE, NT>(c, a, ex, grid); return true;
+    case 32: launch_spmmp<T, V, 32, CPL, 1, OP, MODE, NT>(c, a, ex, grid); return true;
     default: return false;
     }
 }
@@ -425,6 +430,7 @@ bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wol
     ex.partials = partials;
     SpmvArgs b = a;
     b.stop = all_done;
+    if (a.kchunk == -2) return dispatch_spmmp<double, double, SpOp::AX, 2, true>(c, b, ex, nblk);
     return dispatch_spmmp<double, double, SpOp::AX, 2>(c, b, ex, nblk);
 }
 
@@ -566,13 +572,37 @@ static void launchk(Context* c, const SpmvArgs& a) {
     // whole rows of x as 16-byte vectors per lane: CPL = 16 / sizeof(V) columns
     constexpr int CPL = 16 / (int)sizeof(V);
     bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b | (uintptr_t)a.y64) & 15) == 0;
+    if (a.kchunk == -1 && aligned && spmmp_lanes(a.k, CPL)) {
+        // one launch over all rows (no persistent pipeline): the autotuner's other full-width choice
+        int KP = spmmp_lanes(a.k, CPL);
+        bool out64 = OP == SpOp::JACOBI && a.y64 != nullptr;
+#define BCMGX_V(KK, MM)                                                                                               \
+    case KK:                                                                                                          \
+        if (out64)                                                                                                    \
+            k_spmmv<T, V, KK, CPL, MM, OP, true><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(                       \
+                a.n, a.k, a.row, a.col, val, x, y, b, D, al, be, a.stop, a.y64);                                      \
+        else                                                                                                          \
+            k_spmmv<T, V, KK, CPL, MM, OP><<<grid_for(a.n, WG / KK), WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, val, \
+                                                                                        x, y, b, D, al, be, a.stop);  \
+        break;
+        switch (KP) {
+            BCMGX_V(2, 8) BCMGX_V(4, 8) BCMGX_V(8, 4) BCMGX_V(16, 2) BCMGX_V(32, 1)
+        default: throw Error(BCMGX_ERR_ARG, "at most 64 right-hand sides");
+        }
+#undef BCMGX_V
+        BCMGX_LAUNCHED();
+        return;
+    }
     if (kc == a.k && aligned && spmmp_lanes(a.k, CPL)) {
         SpmmExtra ex;
+        bool nt = a.kchunk == -2;
         if (OP == SpOp::JACOBI && a.y64) {
             ex.y64 = a.y64;
-            dispatch_spmmp<T, V, OP, 1>(c, a, ex, 0);
+            if (nt) dispatch_spmmp<T, V, OP, 1, true>(c, a, ex, 0);
+            else dispatch_spmmp<T, V, OP, 1>(c, a, ex, 0);
         } else {
-            dispatch_spmmp<T, V, OP, 0>(c, a, ex, 0);
+            if (nt) dispatch_spmmp<T, V, OP, 0, true>(c, a, ex, 0);
+            else dispatch_spmmp<T, V, OP, 0>(c, a, ex, 0);
         }
         return;
     }

@@ -223,7 +223,9 @@ int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, do
  * dimension ld >= n; the device works on k interleaved columns).  No
  * counterpart in the reference, which multiplies one vector at a time: it is
  * the multi-vector kernel of the batched solve.  kchunk: columns per launch
- * (0 = all k).  Each column equals bcmgx_spmv of that column bit for bit. */
+ * (0 = all k through the pipelined persistent kernel, -1 = all k through the
+ * one-pass kernel, -2 = pipelined with streamed matrix loads).  Each column is the sequential row sum of that column
+ * (the oracle's order) bit for bit. */
 int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
                double beta, int op, int kchunk, int mem);
 /* C = A*B (replaces nsparse spgemm_csrseg_kernel_hash, kernel_spgemm_hash_d.cu:1196-1247),
@@ -242,7 +244,8 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
  * vectors (precision 1); `reps` timed launches on the context stream after one
  * warm-up, bracketed by HIP events.  *ms = average milliseconds per launch.
  * `lanes`: k == 1: lanes per row (0 = autotuned, < 0 = CSR-stream kernel);
- * k > 1: columns per launch (0 = all k). */
+ * k > 1: columns per launch (0 = all k, pipelined kernel; -1 = all k, one-pass kernel;
+ * -2 = all k, pipelined kernel with streamed matrix loads). */
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms);
 /* Sustained device-to-device copy bandwidth (GB/s) over `bytes` bytes. */
 int bcmgx_stream_copy_gbs(bcmgx_context ctx, size_t bytes, int reps, double* gbs);

@@ -86,7 +86,8 @@ SPMM_CASES = {
 
 
 @pytest.mark.parametrize("name", list(SPMM_CASES))
-@pytest.mark.parametrize("k,kchunk", [(2, 0), (3, 0), (8, 0), (13, 4), (32, 0), (32, 8), (64, 0), (64, 16)])
+@pytest.mark.parametrize("k,kchunk", [(2, 0), (3, 0), (4, -1), (8, 0), (8, -2), (13, 4), (16, -1), (32, 0), (32, -1),
+                                     (32, -2), (32, 8), (64, 0), (64, -1), (64, -2), (64, 16)])
 def test_spmm_columns_bit_exact(ctx, name, k, kchunk):
     """Every column of the multi-vector kernel equals the oracle's sequential
     row sum (same order, no contraction) bit for bit."""
