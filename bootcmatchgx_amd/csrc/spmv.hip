@@ -181,6 +181,7 @@ __global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restric
 
 // Operands of the pipelined SpMM's optional epilogues.
 struct SpmmExtra {
+    const int* sched = nullptr;    // row visiting order (a permutation of 0..n-1), or null for 0..n-1
     double* y64 = nullptr;         // MODE 1: JACOBI result widened to FP64
     const double* r = nullptr;     // MODE 2: FCG residual
     const double* wold = nullptr;  // MODE 2: previous A-direction
@@ -225,12 +226,15 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
     T* sv = sval + g * E;
     const V* xc = x + cl * CPL;
     VD da = (VD)0.0, db = (VD)0.0, dc = (VD)0.0;
-    auto bounds = [&](int grp, int& rs, int& re) {
-        int ri = grp * RPB + g;
-        if (grp < g1 && ri < n) {
+    // row of schedule position grp*RPB + g (-1 past the end) and its bounds
+    auto bounds = [&](int grp, int& ri, int& rs, int& re) {
+        int pos = grp * RPB + g;
+        if (grp < g1 && pos < n) {
+            ri = ex.sched ? ex.sched[pos] : pos;
             rs = row[ri];
             re = row[ri + 1];
         } else {
+            ri = -1;
             rs = 0;
             re = 0;
         }
@@ -265,15 +269,15 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
         }
     };
     int grp = g0 + local;  // uniform over the block
-    int s0, e0, s1, e1;
-    bounds(grp, s0, e0);
-    bounds(grp + nloc, s1, e1);
+    int r0, s0, e0, r1, s1, e1;
+    bounds(grp, r0, s0, e0);
+    bounds(grp + nloc, r1, s1, e1);
     int pc[M];
     T pv[M];
     chunk(s0, e0, pc, pv);
     while (grp < g1) {
-        int s2, e2;
-        bounds(grp + 2 * nloc, s2, e2);
+        int r2, s2, e2;
+        bounds(grp + 2 * nloc, r2, s2, e2);
 #pragma unroll
         for (int m = 0; m < M; m++) {
             sc[m * KP + cl] = pc[m];
@@ -299,8 +303,8 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
             gather(e0 - base < E ? e0 - base : E, t);
             wave_lds_sync();
         }
-        int ri = grp * RPB + g;
-        if (ri < n && active) {
+        int ri = r0;
+        if (ri >= 0 && active) {
             long long o = (long long)ri * k + cl * CPL;
             if constexpr (MODE == 1) {
                 VV res = ((-t + *(const VV*)(b + o)) / D[ri]) + *(const VV*)(x + o);
@@ -320,8 +324,8 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
             }
         }
         grp += nloc;
-        s0 = s1; e0 = e1;
-        s1 = s2; e1 = e2;
+        r0 = r1; s0 = s1; e0 = e1;
+        r1 = r2; s1 = s2; e1 = e2;
 #pragma unroll
         for (int m = 0; m < M; m++) {
             pc[m] = pc1[m];
@@ -425,6 +429,7 @@ bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wol
     bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)r | (uintptr_t)wold) & 15) == 0;
     if (nblk <= 0 || nblk % 8 || !aligned || !spmmp_lanes(a.k, 2) || (a.kchunk > 0 && a.kchunk < a.k)) return false;
     SpmmExtra ex;
+    ex.sched = a.sched;
     ex.r = r;
     ex.wold = wold;
     ex.partials = partials;
@@ -595,6 +600,7 @@ static void launchk(Context* c, const SpmvArgs& a) {
     }
     if (kc == a.k && aligned && spmmp_lanes(a.k, CPL)) {
         SpmmExtra ex;
+        ex.sched = a.sched;
         bool nt = a.kchunk == -2;
         if (OP == SpOp::JACOBI && a.y64) {
             ex.y64 = a.y64;
@@ -677,11 +683,25 @@ __global__ void k_jacobi0(long long nk, int k, const V* __restrict__ f, const V*
     // ((-(A 0) + f) / D) + 0 == f / D
     x[t] = f[t] / D[t / k];
 }
+// 32-bit indexing (n k < 2^31); k == 1 needs no division
+template <typename V, bool K1>
+__global__ void k_jacobi0_32(int nk, int k, const V* __restrict__ f, const V* __restrict__ D, V* __restrict__ x,
+                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    x[t] = f[t] / D[K1 ? t : t / k];
+}
 template <typename V>
 void jacobi_from_zero(Context* c, int n, int k, const V* f, const V* D, V* x, const int* stop) {
     long long nk = (long long)n * k;
     if (!nk) return;
-    k_jacobi0<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, f, D, x, stop);
+    if (nk < (1ll << 31)) {
+        if (k == 1) k_jacobi0_32<V, true><<<grid_for(nk, WG), WG, 0, c->stream>>>((int)nk, k, f, D, x, stop);
+        else k_jacobi0_32<V, false><<<grid_for(nk, WG), WG, 0, c->stream>>>((int)nk, k, f, D, x, stop);
+    } else {
+        k_jacobi0<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, f, D, x, stop);
+    }
     BCMGX_LAUNCHED();
 }
 template void jacobi_from_zero<double>(Context*, int, int, const double*, const double*, double*, const int*);
@@ -714,11 +734,32 @@ __global__ void k_prolong(long long nk, int k, const int* __restrict__ agg, cons
     V tt = pval[i] * xc[(long long)agg[i] * k + cc];
     x[t] = tt + x[t];
 }
+// the same with CPL consecutive columns per thread (16-byte accesses), 32-bit indexing
+template <typename V, int CPL>
+__global__ void k_prolong_v(int nkv, int kv, const int* __restrict__ agg, const V* __restrict__ pval,
+                            const V* __restrict__ xc, V* __restrict__ x, const int* __restrict__ stop) {
+    typedef V VV __attribute__((ext_vector_type(CPL)));
+    if (stop && *stop) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nkv) return;
+    int i = t / kv;
+    int cv = t - i * kv;
+    VV xv = *((const VV*)xc + ((long long)agg[i] * kv + cv));
+    VV* xo = (VV*)x + t;
+    *xo = pval[i] * xv + *xo;
+}
 template <typename V>
 void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const V* xc, V* x, const int* stop) {
     long long nk = (long long)n * k;
     if (!nk) return;
-    k_prolong<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, agg, pval, xc, x, stop);
+    constexpr int CPL = 16 / (int)sizeof(V);
+    bool aligned = (((uintptr_t)xc | (uintptr_t)x) & 15) == 0;
+    if (k % CPL == 0 && aligned && nk / CPL < (1ll << 31)) {
+        int nkv = (int)(nk / CPL);
+        k_prolong_v<V, CPL><<<grid_for(nkv, WG), WG, 0, c->stream>>>(nkv, k / CPL, agg, pval, xc, x, stop);
+    } else {
+        k_prolong<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, agg, pval, xc, x, stop);
+    }
     BCMGX_LAUNCHED();
 }
 template void prolong_add<double>(Context*, int, int, const int*, const double*, const double*, double*, const int*);
