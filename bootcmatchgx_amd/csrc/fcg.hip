@@ -14,6 +14,8 @@
 
 namespace bcmgx {
 
+// partial-sum blocks of the streaming FCG kernels: a function of n only, so
+// the reduction order never changes
 int fcg_partial_blocks(int n) {
     long long b = ((long long)n + 2047) / 2048;
     if (b < 1) b = 1;

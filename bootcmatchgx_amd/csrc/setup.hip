@@ -607,11 +607,17 @@ __global__ void k_count_products(int n, const int* __restrict__ arow, const int*
 
 // bins: 0..2 rows of <= 16 / 32 / 64 products (W-lane sorts), 3 medium, 4 large
 constexpr int NBINS = 5;
-// bin rows by product count; one atomic per wave and bin (order inside a bin is irrelevant)
+// bin rows by product count (order inside a bin is irrelevant): the waves of a
+// block take slots from LDS counters, then one global atomic per non-empty
+// bin per block reserves the block's range
 __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ arow, const long long* __restrict__ cnt,
                                                  int* __restrict__ bins, int* __restrict__ bincount, int cap) {
+    __shared__ int lcount[NBINS];
+    __shared__ int lbase[NBINS];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int lane = threadIdx.x & 63;
+    if (threadIdx.x < NBINS) lcount[threadIdx.x] = 0;
+    __syncthreads();
     int b = -1;
     if (i < n) {
         long long p = cnt[i];
@@ -620,15 +626,20 @@ __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ 
         b = m <= 16 ? 0 : m <= 32 ? 1 : m <= SMALL_P ? 2 : (m <= MED_P ? 3 : 4);
     }
     unsigned long long below = (1ull << lane) - 1ull;
+    int myslot = 0;
     for (int bb = 0; bb < NBINS; bb++) {
-        unsigned long long m = __ballot(b == bb);
-        if (!m) continue;
-        int leader = __ffsll((long long)m) - 1;
+        unsigned long long msk = __ballot(b == bb);
+        if (!msk) continue;
+        int leader = __ffsll((long long)msk) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(&bincount[bb], __popcll(m));
+        if (lane == leader) base = atomicAdd(&lcount[bb], __popcll(msk));
         base = __shfl(base, leader, 64);
-        if (b == bb) bins[(long long)bb * cap + base + __popcll(m & below)] = i;
+        if (b == bb) myslot = base + __popcll(msk & below);
     }
+    __syncthreads();
+    if (threadIdx.x < NBINS) lbase[threadIdx.x] = lcount[threadIdx.x] ? atomicAdd(&bincount[threadIdx.x], lcount[threadIdx.x]) : 0;
+    __syncthreads();
+    if (b >= 0) bins[(long long)b * cap + lbase[b] + myslot] = i;
 }
 
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m, int w) {
