@@ -245,7 +245,7 @@ def main():
         "stream_copy_gbs": copy_gbs,
         "autograd_forward_adjoint": adjoint,
         "roofline": {
-            "kernel": "k_spmv_stream<double,double,TPR,AX> (autotuned CSR-stream SpMV, fine 27-point matrix, FP64)",
+            "kernel": "autotuned CSR SpMV (k_spmv_stream / k_spmv1), fine 27-point matrix, FP64",
             "bound": "hbm",
             "achieved": spmv_gbs,
             "peak": HBM_PEAK_GBS,

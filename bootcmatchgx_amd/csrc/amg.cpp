@@ -34,7 +34,9 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     if (it != c->tuned_lanes.end()) return it->second;
     int base = default_lanes(A.nnz, A.n);
     // vector kernels with L lanes per row, and CSR-stream kernels (negative: lanes per row reduction)
-    int cands[6] = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8};
+    // vector kernels (L lanes per row), CSR-stream kernels (-TPR lanes per row
+    // reduction) and their paired-load form (-200 - TPR)
+    int cands[8] = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8, -204, -208};
     hipEvent_t e0, e1;
     BCMGX_HIP(hipEventCreate(&e0));
     BCMGX_HIP(hipEventCreate(&e1));

@@ -516,6 +516,105 @@ __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict
     }
 }
 
+// The CSR-stream SpMV with the matrix streamed in pairs: every thread loads
+// two consecutive (val, col) entries with one 2-wide load each (16 + 8 bytes
+// for FP64), from an even-aligned start, so a block issues half as many
+// memory instructions.  Same chunks, staging and per-row reduction as
+// k_spmv_stream, hence the same bits.  The final pair of the matrix is read
+// element by element when nnz is odd (no read past the arrays).
+template <typename T, typename V, int TPR, SpOp OP>
+__global__ __launch_bounds__(WG) void k_spmv_stream2(int n, long long nnz, const int* __restrict__ row,
+                                                     const int* __restrict__ col, const T* __restrict__ val,
+                                                     const V* __restrict__ x, V* __restrict__ y, const V* __restrict__ b,
+                                                     const V* __restrict__ D, V alpha, V beta,
+                                                     const int* __restrict__ stop) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    typedef int I2 __attribute__((ext_vector_type(2)));
+    if (stop && *stop) return;
+    constexpr int RB = WG / TPR;
+    constexpr int CAP = RB * 32 < 4096 ? RB * 32 : 4096;
+    constexpr int UP = 4;  // pairs per thread per pass
+    __shared__ V prod[CAP];
+    __shared__ int rp[RB + 1];
+    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int r0 = blk * RB;
+    int tid = threadIdx.x;
+    int rend = r0 + RB < n ? r0 + RB : n;
+    int base = row[r0], cnt = row[rend] - base;
+    int myrp = 0;
+    if (tid < RB) myrp = row[r0 + tid < n ? r0 + tid : n];
+    bool staged = cnt <= CAP;
+    if (staged) {
+        int abase = base & ~1;             // even global start: 2-wide loads stay aligned
+        int acnt = base + cnt - abase;     // entries from abase to the chunk end
+        const T2* v2 = (const T2*)(val + abase);
+        const I2* c2 = (const I2*)(col + abase);
+        for (int p0 = tid; 2 * p0 < acnt; p0 += UP * WG) {
+            T2 a[UP];
+            I2 cc[UP];
+#pragma unroll
+            for (int q = 0; q < UP; q++) {
+                int pp = p0 + q * WG;
+                a[q] = (T2)(T)0;
+                cc[q] = (I2)0;
+                if (2 * pp < acnt) {
+                    long long g = (long long)abase + 2 * pp;
+                    if (g + 1 < nnz) {
+                        a[q] = __builtin_nontemporal_load(v2 + pp);
+                        cc[q] = __builtin_nontemporal_load(c2 + pp);
+                    } else {  // the matrix's last entry, unpaired
+                        a[q].x = val[g];
+                        cc[q].x = col[g];
+                    }
+                }
+            }
+            V x0[UP], x1[UP];
+#pragma unroll
+            for (int q = 0; q < UP; q++) {
+                int t = abase + 2 * (p0 + q * WG) - base;  // chunk position of the pair's first entry (may be -1)
+                x0[q] = (t >= 0 && t < cnt) ? x[cc[q].x] : (V)0;
+                x1[q] = (t + 1 >= 0 && t + 1 < cnt) ? x[cc[q].y] : (V)0;
+            }
+#pragma unroll
+            for (int q = 0; q < UP; q++) {
+                int t = abase + 2 * (p0 + q * WG) - base;
+                V a0 = (V)a[q].x, a1 = (V)a[q].y;
+                if (OP == SpOp::AXPBY) {
+                    a0 = alpha * a0;
+                    a1 = alpha * a1;
+                }
+                if (t >= 0 && t < cnt) prod[t] = a0 * x0[q];
+                if (t + 1 >= 0 && t + 1 < cnt) prod[t + 1] = a1 * x1[q];
+            }
+        }
+    }
+    if (tid < RB) rp[tid] = myrp;
+    if (tid == 0) rp[RB] = base + cnt;  // = row[rend]
+    __syncthreads();
+    int lr = tid / TPR, sub = tid % TPR;
+    int r = r0 + lr;
+    if (r >= n) return;  // whole TPR-lane group
+    int s = rp[lr], e = rp[lr + 1];
+    V acc = 0;
+    if (staged) {
+        for (int q = s + sub; q < e; q += TPR) acc += prod[q - base];
+    } else {
+        for (int q = s + sub; q < e; q += TPR) {
+            V a = (V)val[q];
+            if (OP == SpOp::AXPBY) a = alpha * a;
+            acc += a * x[col[q]];
+        }
+    }
+#pragma unroll
+    for (int o = TPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, TPR);
+    if (sub == 0) {
+        if (OP == SpOp::AX) y[r] = acc;
+        else if (OP == SpOp::RES) y[r] = b[r] - acc;
+        else if (OP == SpOp::AXPBY) y[r] = acc + beta * y[r];
+        else y[r] = ((-acc + b[r]) / D[r]) + x[r];
+    }
+}
+
 int default_lanes(long long nnz, int n) {
     if (n <= 0) return 1;
     double d = (double)nnz / n;
@@ -536,6 +635,24 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
     const V* b = (const V*)a.b;
     const V* D = (const V*)a.D;
     V al = (V)a.alpha, be = (V)a.beta;
+    if (L <= -200) {  // CSR-stream variant streaming (val, col) pairs, TPR = -L - 200 lanes per row
+        bool aligned = (((uintptr_t)a.val) & (2 * sizeof(T) - 1)) == 0 && (((uintptr_t)a.col) & 7) == 0;
+        if (!aligned) L += 200;  // the single-entry stream kernel serves unaligned arrays
+    }
+    if (L <= -200) {
+#define BCMGX_S2(TT)                                                                                           \
+    case TT:                                                                                                   \
+        k_spmv_stream2<T, V, TT, OP><<<grid_for(a.n, WG / TT), WG, 0, c->stream>>>(a.n, a.nnz, a.row, a.col, val, x, \
+                                                                                  y, b, D, al, be, a.stop);    \
+        break;
+        switch (-L - 200) {
+            BCMGX_S2(1) BCMGX_S2(2) BCMGX_S2(4) BCMGX_S2(8) BCMGX_S2(16)
+        default: throw Error(BCMGX_ERR_ARG, "stream lanes per row must be a power of two <= 16");
+        }
+#undef BCMGX_S2
+        BCMGX_LAUNCHED();
+        return;
+    }
     if (L < 0) {  // CSR-stream variant, TPR = -L lanes per row
 #define BCMGX_S(TT)                                                                                                 \
     case TT:                                                                                                        \

@@ -216,7 +216,8 @@ int bcmgx_dump_info(const char* path, const bcmgx_params* p, bcmgx_prec prec, co
 /* ---- kernel-level entry points (parity tests and micro-benchmarks) --------
  * y = alpha*A*x + beta*y, op as _CSR_vector_mul_mini_warp<OP> (CSR.cu:452-492):
  * op 1: y = A x; op 2: y = y - A x; op 0: y = alpha A x + beta y.
- * lanes: lanes per row (0 = autotuned choice). */
+ * lanes: lanes per row (0 = autotuned choice; -T: CSR-stream kernel with T lanes
+ * per row reduction; -200-T: the same streaming (val, col) pairs). */
 int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, double alpha,
                double beta, int op, int lanes, int mem);
 /* The same on k right-hand sides at once (x, y column-major with leading
@@ -243,7 +244,7 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
  * y = A x on k interleaved columns, FP64 (precision 0) or FP32 matrix and
  * vectors (precision 1); `reps` timed launches on the context stream after one
  * warm-up, bracketed by HIP events.  *ms = average milliseconds per launch.
- * `lanes`: k == 1: lanes per row (0 = autotuned, < 0 = CSR-stream kernel);
+ * `lanes`: k == 1: lanes per row (0 = autotuned, -T = CSR-stream kernel, -200-T = paired-load CSR-stream);
  * k > 1: columns per launch (0 = all k, pipelined kernel; -1 = all k, one-pass kernel;
  * -2 = all k, pipelined kernel with streamed matrix loads). */
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms);

@@ -63,7 +63,7 @@ def oparams(p):
 
 # ------------------------------------------------------------------- SpMV
 @pytest.mark.parametrize("name", ["lap7_12", "lap27_10", "rand"])
-@pytest.mark.parametrize("lanes", [0, 1, 4, 16, 64, -1, -2, -4, -16])
+@pytest.mark.parametrize("lanes", [0, 1, 4, 16, 64, -1, -2, -4, -16, -201, -204, -208, -216])
 def test_spmv_vs_oracle(ctx, name, lanes):
     oa = CASES[name]()
     A = dev(ctx, oa)
