@@ -82,13 +82,12 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
 }
 
 template <typename V>
-static float time_spmm(Context* c, const DCsr& A, const void* val, int k, int chunk, const int* sched, hipEvent_t e0,
-                       hipEvent_t e1) {
+static float time_spmm(Context* c, const DCsr& A, const void* val, int k, int chunk, hipEvent_t e0, hipEvent_t e1) {
     DBuf<V> x(c, (size_t)A.m * k), y(c, (size_t)A.n * k);
     fill<V>(c, x.get(), (long long)A.m * k, (V)1);
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = val; a.nnz = A.nnz;
-    a.x = x.get(); a.y = y.get(); a.k = k; a.kchunk = chunk; a.sched = sched;
+    a.x = x.get(); a.y = y.get(); a.k = k; a.kchunk = chunk;
     spmv<V, V>(c, SpOp::AX, a);
     BCMGX_HIP(hipEventRecord(e0, c->stream));
     for (int r = 0; r < 3; r++) spmv<V, V>(c, SpOp::AX, a);
@@ -99,11 +98,10 @@ static float time_spmm(Context* c, const DCsr& A, const void* val, int k, int ch
     return ms;
 }
 
-int tune_chunk(Context* c, const DCsr& A, bool fp32, int k, const int* sched) {
+int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
     if (k <= 4 || A.n < 65536) return k;
     if (fp32 && !A.valf) throw Error(BCMGX_ERR_STATE, "tune_chunk: FP32 values not built");
-    std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d") + ":k" + std::to_string(k) +
-                      (sched ? "s" : "");
+    std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d") + ":k" + std::to_string(k);
     auto it = c->tuned_lanes.find(sig);
     if (it != c->tuned_lanes.end()) return it->second;
     hipEvent_t e0, e1;
@@ -119,8 +117,8 @@ int tune_chunk(Context* c, const DCsr& A, bool fp32, int k, const int* sched) {
         if (ch == 4) break;
     }
     for (int ch : cands) {
-        float ms = fp32 ? time_spmm<float>(c, A, A.valf.get(), k, ch, sched, e0, e1)
-                        : time_spmm<double>(c, A, A.val.get(), k, ch, sched, e0, e1);
+        float ms = fp32 ? time_spmm<float>(c, A, A.valf.get(), k, ch, e0, e1)
+                        : time_spmm<double>(c, A, A.val.get(), k, ch, e0, e1);
         if (ms < bestms) { bestms = ms; best = ch; }
     }
     (void)hipEventDestroy(e0);
@@ -223,33 +221,6 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
     H.avg_cratio = nlv > 1 ? avc / (nlv - 1) : 1;
 }
 
-// SpMM visiting order of every large level: starting from the coarsest level
-// in index order, each coarse row is replaced by the rows of its aggregate
-// (the restrictor's row, ascending), level by level down.  On a regular grid
-// this is a recursive 2x2x2 blocking (a Morton-like order inside the cells of
-// the coarsest grid), so the rows an XCD works on at once gather x from a
-// compact, L2-resident neighbourhood.  Only the order of independent row sums
-// changes; the order is a function of the hierarchy, hence deterministic.
-static void build_schedules(Context* c, Hierarchy& H) {
-    int nl = H.levels();
-    if (nl < 2) return;
-    // coarse-to-fine: sched of level l is the expansion of level l+1's order
-    DBuf<int> top(c, (size_t)std::max(H.L[nl - 1]->mat().n, 1));
-    fill_iota(c, top.get(), H.L[nl - 1]->mat().n);
-    const int* in = top.get();
-    for (int l = nl - 2; l >= 0; l--) {
-        Level& lv = *H.L[l];
-        int n = lv.mat().n;
-        lv.sched.reset(c, (size_t)std::max(n, 1));
-        expand_schedule(c, in, H.L[l + 1]->mat().n, lv.R, lv.sched.get());
-        in = lv.sched.get();
-    }
-    c->sync();
-    // small levels run one-pass: their schedules are not needed
-    for (int l = 0; l + 1 < nl; l++)
-        if (H.L[l]->mat().n < 65536) H.L[l]->sched.release();
-}
-
 static void make_fp32_copies(Context* c, Hierarchy& H) {
     for (auto& lp : H.L) {
         Level& l = *lp;
@@ -285,7 +256,6 @@ struct SolveWork {
     int lanesA = 0;
     int chunkA = 0;
     int mv_blocks = 0;  // grid of the fused SpMM + dots kernel (0: not used)
-    const int* sched = nullptr;  // row order of the outer SpMM (the hierarchy's level-0 schedule)
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -305,7 +275,7 @@ void Prec::ensure_work(int k) {
     if (ptype == BCMGX_PREC_BCMG && k > 1 && k != tuned_k) {
         // SpMM column chunk for this batch width (tuned outside any graph capture)
         bool f32 = p.precision == BCMGX_PRECISION_FP32;
-        for (auto& lp : H.L) lp->chunkA = p.autotune ? tune_chunk(ctx, lp->mat(), f32, k, lp->sched.get()) : k;
+        for (auto& lp : H.L) lp->chunkA = p.autotune ? tune_chunk(ctx, lp->mat(), f32, k) : k;
         tuned_k = k;
     }
     if (k <= kcap) return;
@@ -350,8 +320,7 @@ static const V* mat_val(const DCsr& A) {
 // `outv` it writes it there (same type) and returns outv.
 template <typename V>
 static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const V* F, V* cur, V* oth, int s, bool xzero,
-                        int k, const int* stop, int chunk = 0, double* out64 = nullptr, V* outv = nullptr,
-                        const int* sched = nullptr) {
+                        int k, const int* stop, int chunk = 0, double* out64 = nullptr, V* outv = nullptr) {
     int done = 0;
     if (s <= 0) {
         if (xzero) fill<V>(c, cur, (long long)A.n * k, (V)0);
@@ -365,7 +334,6 @@ static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const 
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
         a.x = cur; a.y = oth; a.b = F; a.D = M; a.k = k; a.lanes = lanes; a.stop = stop; a.kchunk = chunk;
-        a.sched = sched;
         bool last = done + 1 == s;
         if (last && out64) {
             a.y64 = out64;
@@ -442,10 +410,10 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
         cur = w.X.get();
         oth = w.X2.get();
         cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff - 1, false, k, R.stop,
-                               lv.chunkA, nullptr, nullptr, lv.sched.get());
+                               lv.chunkA);
     } else {
         cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.prerelax_sweeps * coeff, xzero, k, R.stop,
-                               lv.chunkA, nullptr, nullptr, lv.sched.get());
+                               lv.chunkA);
     }
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
     // Res = F - A x ; F_{l+1} = R Res
@@ -453,7 +421,6 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
         a.x = cur; a.y = w.Res.get(); a.b = F; a.k = k; a.lanes = lv.lanesA; a.stop = R.stop; a.kchunk = lv.chunkA;
-        a.sched = lv.sched.get();
         spmv<V, V>(c, SpOp::RES, a);
     }
     LevelWork<V>& wc = (*R.W)[l + 1];
@@ -476,7 +443,7 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
     if (xc) prolong_add<V>(c, A.n, k, lv.agg.get(), lvl_pval<V>(lv), xc, cur, R.stop);
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
     cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.postrelax_sweeps * coeff, false, k, R.stop,
-                           lv.chunkA, out64, outv, lv.sched.get());
+                           lv.chunkA, out64, outv);
     return cur;
 }
 
@@ -576,7 +543,6 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
                     "only the MULTIPLICATIVE composition is applied (bcmg.cu:60-87)");
         BCMGX_CHECK(p.cycle_type >= 0 && p.cycle_type < BCMGX_CYCLE_INVALID, BCMGX_ERR_PARAM, "invalid cycle_type");
         build_hierarchy(c, A, p, pr->H);
-        build_schedules(c, pr->H);
         bool f32 = p.precision == BCMGX_PRECISION_FP32;
         if (f32) make_fp32_copies(c, pr->H);
         for (auto& lp : pr->H.L) {
@@ -607,7 +573,7 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     prec->apply(s.r.get(), P0, s.k, true, stop, s.fused);  // P0 zero-filled in the reference (FCG.cu:106,119)
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
-    a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA; a.sched = s.sched;
+    a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
     if (s.mv_blocks > 0 && spmm_dots(c, a, s.r.get(), W1, s.fb.partials, s.mv_blocks, s.fb.all_done)) {
         fcg_scalars3(c, s.fb, s.mv_blocks);
     } else {
@@ -645,10 +611,8 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         s.fb.partials = s.partials.get();
         s.fb.st = s.st.get();
         s.fb.all_done = s.flags.get();
-        if (prec->ptype == BCMGX_PREC_BCMG && !prec->H.L.empty() && &prec->H.L[0]->mat() == &A)
-            s.sched = prec->H.L[0]->sched.get();
         s.lanesA = p.autotune ? tune_lanes(c, A, false) : default_lanes(A.nnz, A.n);
-        s.chunkA = p.autotune ? tune_chunk(c, A, false, k, s.sched) : k;
+        s.chunkA = p.autotune ? tune_chunk(c, A, false, k) : k;
     }
     SolveWork& s = *sp;
     s.fb.rtol = p.rtol;

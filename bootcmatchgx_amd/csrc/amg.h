@@ -20,7 +20,6 @@ struct Level {
     DCsr R;                 // restrictor P^T (rows sorted by fine index)
     int lanesA = 0, lanesR = 0;
     int chunkA = 0;         // columns per SpMM launch for the current batch width
-    DBuf<int> sched;        // SpMM row visiting order (aggregate-grouped), empty for small levels
     const DCsr& mat() const { return Aref ? *Aref : A; }
 };
 
@@ -88,7 +87,6 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
 // autotune: fastest lanes per row for SpMV with A (k = 1)
 int tune_lanes(Context* c, const DCsr& A, bool fp32);
 // autotune: fastest column chunk for SpMM with A on k interleaved columns
-// (visiting the rows in `sched` order when given)
-int tune_chunk(Context* c, const DCsr& A, bool fp32, int k, const int* sched = nullptr);
+int tune_chunk(Context* c, const DCsr& A, bool fp32, int k);
 
 }  // namespace bcmgx

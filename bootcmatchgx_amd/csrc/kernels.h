@@ -47,7 +47,6 @@ struct SpmvArgs {
                                 // kernel; -2 = all k, pipelined kernel with streamed matrix loads)
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
     double* y64 = nullptr;      // JACOBI with V = float: write the result widened to FP64 here (y is scratch)
-    const int* sched = nullptr; // k > 1: order in which the pipelined SpMM visits the rows (a permutation), or null
 };
 // T = matrix value type, V = vector type: (double,double), (float,float), (float,double)
 template <typename T, typename V>
@@ -137,12 +136,6 @@ void transpose(Context* c, const DCsr& A, DCsr& T);
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
 // P <- P P_i for one-entry-per-row operators: agg[k] = agg_i[agg[k]], pval[k] *= pval_i[agg[k]]
 void compose_prolongators(Context* c, int n, int* agg, double* pval, const int* agg_i, const double* pval_i);
-// one step of a top-down visiting order through the hierarchy: `in` lists the
-// m rows of a coarse level in order (m = R.n); `out` (R.m entries) lists the
-// fine rows, each coarse row replaced by its aggregate's rows as R stores them
-void expand_schedule(Context* c, const int* in, int m, const DCsr& R, int* out);
-// p[i] = i, i < n
-void fill_iota(Context* c, int* p, int n);
 // csr of a one-entry-per-row prolongator (row = identity)
 void prolongator_csr(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& P);
 // g[k] = -sum_c lambda[row(k), c] x[col(k), c]

@@ -476,11 +476,6 @@ __global__ void k_iota(int n, int* __restrict__ r) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= n) r[i] = i;
 }
-void fill_iota(Context* c, int* p, int n) {
-    if (n <= 0) return;
-    k_iota<<<grid_for(n, WG), WG, 0, c->stream>>>(n - 1, p);  // p[i] = i for i <= n - 1
-    BCMGX_LAUNCHED();
-}
 void prolongator_csr(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& P) {
     P.alloc(c, n, nc, n);
     k_iota<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, P.row.get());
@@ -551,33 +546,6 @@ void transpose_prolongator(Context* c, int n, int nc, const int* agg, const doub
     k_iota<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, row.get());
     BCMGX_LAUNCHED();
     transpose_raw(c, n, nc, n, row.get(), agg, pval, R);
-}
-
-// one level of the top-down expansion: `in` lists the m coarse rows in visiting
-// order; every coarse row is replaced by its fine rows (R's row, ascending)
-__global__ void k_expand_count(int m, const int* __restrict__ in, const int* __restrict__ Rrow, int* __restrict__ cnt) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < m) cnt[t] = Rrow[in[t] + 1] - Rrow[in[t]];
-}
-__global__ void k_expand_scatter(int m, const int* __restrict__ in, const int* __restrict__ Rrow, const int* __restrict__ Rcol,
-                                 const int* __restrict__ off, int* __restrict__ out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    int s = Rrow[in[t]], e = Rrow[in[t] + 1];
-    int o = off[t];
-    for (int j = s; j < e; j++) out[o + (j - s)] = Rcol[j];
-}
-
-void expand_schedule(Context* c, const int* in, int m, const DCsr& R, int* out) {
-    BCMGX_CHECK(R.n == m, BCMGX_ERR_STATE, "expand_schedule: restrictor does not match the coarse order");
-    if (m <= 0) return;
-    DBuf<int> cnt(c, (size_t)m), off(c, (size_t)m + 1);
-    k_expand_count<<<grid_for(m, WG), WG, 0, c->stream>>>(m, in, R.row.get(), cnt.get());
-    BCMGX_LAUNCHED();
-    int total = scan_exclusive(c, cnt.get(), off.get(), m);
-    BCMGX_CHECK(total == R.m, BCMGX_ERR_STATE, "expand_schedule: restrictor does not cover every fine row once");
-    k_expand_scatter<<<grid_for(m, WG), WG, 0, c->stream>>>(m, in, R.row.get(), R.col.get(), off.get(), out);
-    BCMGX_LAUNCHED();
 }
 
 // ---------------------------------------------------------------- SpGEMM ---

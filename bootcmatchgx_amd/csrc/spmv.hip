@@ -181,7 +181,6 @@ __global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restric
 
 // Operands of the pipelined SpMM's optional epilogues.
 struct SpmmExtra {
-    const int* sched = nullptr;    // row visiting order (a permutation of 0..n-1), or null for 0..n-1
     double* y64 = nullptr;         // MODE 1: JACOBI result widened to FP64
     const double* r = nullptr;     // MODE 2: FCG residual
     const double* wold = nullptr;  // MODE 2: previous A-direction
@@ -226,11 +225,11 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
     T* sv = sval + g * E;
     const V* xc = x + cl * CPL;
     VD da = (VD)0.0, db = (VD)0.0, dc = (VD)0.0;
-    // row of schedule position grp*RPB + g (-1 past the end) and its bounds
+    // row grp*RPB + g (-1 past the end) and its bounds
     auto bounds = [&](int grp, int& ri, int& rs, int& re) {
         int pos = grp * RPB + g;
         if (grp < g1 && pos < n) {
-            ri = ex.sched ? ex.sched[pos] : pos;
+            ri = pos;
             rs = row[ri];
             re = row[ri + 1];
         } else {
@@ -429,7 +428,6 @@ bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wol
     bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)r | (uintptr_t)wold) & 15) == 0;
     if (nblk <= 0 || nblk % 8 || !aligned || !spmmp_lanes(a.k, 2) || (a.kchunk > 0 && a.kchunk < a.k)) return false;
     SpmmExtra ex;
-    ex.sched = a.sched;
     ex.r = r;
     ex.wold = wold;
     ex.partials = partials;
@@ -717,7 +715,6 @@ static void launchk(Context* c, const SpmvArgs& a) {
     }
     if (kc == a.k && aligned && spmmp_lanes(a.k, CPL)) {
         SpmmExtra ex;
-        ex.sched = a.sched;
         bool nt = a.kchunk == -2;
         if (OP == SpOp::JACOBI && a.y64) {
             ex.y64 = a.y64;

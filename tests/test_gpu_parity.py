@@ -268,9 +268,9 @@ def test_prec_apply_matches_oracle(ctx):
         assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), prec
 
 
-def test_batched_prec_apply_with_row_schedule_matches_oracle(ctx):
-    """Above 65536 rows the batched SpMM visits level 0 in aggregate order; the
-    cycle's columns must still match the oracle's one-column cycle."""
+def test_batched_prec_apply_large_level_matches_oracle(ctx):
+    """A level above 65536 rows (autotuned persistent SpMM kernels): every
+    column of the batched cycle matches the oracle's one-column cycle."""
     oa = O.laplacian3d(27, 48, 48, 48)
     A = dev(ctx, oa)
     R = np.random.default_rng(15).normal(size=(A.n, 8))
