@@ -426,7 +426,7 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
     LevelWork<V>& wc = (*R.W)[l + 1];
     {
         SpmvArgs a;
-        a.n = lv.R.n; a.m = lv.R.m; a.row = lv.R.row.get(); a.col = lv.R.col.get(); a.val = mat_val<V>(lv.R); a.nnz = lv.R.nnz;
+        a.n = lv.R.n; a.row = lv.R.row.get(); a.col = lv.R.col.get(); a.val = mat_val<V>(lv.R); a.nnz = lv.R.nnz;
         a.x = w.Res.get(); a.y = wc.F.get(); a.k = k; a.lanes = lv.lanesR; a.stop = R.stop;
         spmv<V, V>(c, SpOp::AX, a);
     }

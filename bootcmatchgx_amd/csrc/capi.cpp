@@ -605,7 +605,7 @@ int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, in
     stage_in(c, x, ld, mem, M.m, k, X.get(), tmp);
     if (op != 1) stage_in(c, y, ld, mem, M.n, k, Y.get(), tmp);
     SpmvArgs a;
-    a.n = M.n; a.m = M.m; a.row = M.row.get(); a.col = M.col.get(); a.val = M.val.get(); a.nnz = M.nnz;
+    a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.val = M.val.get(); a.nnz = M.nnz;
     a.x = X.get(); a.y = Y.get(); a.b = Y.get(); a.alpha = alpha; a.beta = beta; a.k = k; a.kchunk = kchunk;
     a.lanes = k == 1 ? tune_lanes(c, M, false) : 0;
     SpOp o = op == 1 ? SpOp::AX : (op == 2 ? SpOp::RES : SpOp::AXPBY);
@@ -624,7 +624,7 @@ int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int
     DCsr& M = A->A;
     size_t nx = (size_t)std::max(M.m, 1) * k, ny = (size_t)std::max(M.n, 1) * k;
     SpmvArgs a;
-    a.n = M.n; a.m = M.m; a.row = M.row.get(); a.col = M.col.get(); a.nnz = M.nnz; a.k = k;
+    a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.nnz = M.nnz; a.k = k;
     if (precision == 1 && !M.valf) {  // the FP32 copy must exist before any FP32 launch (tuning included)
         M.valf.reset(c, std::max(M.nnz, 1));
         convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);

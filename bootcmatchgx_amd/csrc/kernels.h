@@ -31,7 +31,6 @@ enum class SpOp {
 
 struct SpmvArgs {
     int n = 0;
-    int m = 0;                  // columns (x rows); 0 = n
     const int* row = nullptr;
     const int* col = nullptr;
     const void* val = nullptr;  // T*

@@ -31,6 +31,7 @@ void* DevicePool::alloc(size_t bytes) {
         p = it->second;
         free_.erase(it);
     } else {
+        (void)hipSetDevice(device_);  // blocks live on the pool's device
         hipError_t e = hipMalloc(&p, c);
         if (e != hipSuccess) {
             // drop the cached blocks and retry once
@@ -59,6 +60,7 @@ void DevicePool::free(void* p) {
 }
 
 void DevicePool::release_all() {
+    if (!free_.empty() || !live_.empty()) (void)hipSetDevice(device_);
     for (auto& f : free_) (void)hipFree(f.second);
     free_.clear();
     for (auto& l : live_) (void)hipFree(l.first);
