@@ -215,7 +215,9 @@ __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wr
     int ngroups = gridDim.x * (WG / G);
     int total = in_list ? *in_count : n;
     for (int it = (blockIdx.x * WG + (int)threadIdx.x) / G; it < total; it += ngroups) {
-        int cur = in_list ? in_list[it] : it;
+        // round 0 visits the vertices from the highest id down: a suitee prefers
+        // the higher suitor, so offers made in this order are rarely displaced
+        int cur = in_list ? in_list[it] : n - 1 - it;
         int steps = 0;
         while (cur >= 0) {  // uniform inside the group
             if (steps == maxsteps) {
@@ -375,10 +377,17 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
         BCMGX_LAUNCHED();                                                                                              \
         k_tkey<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), tkey.get());     \
         BCMGX_LAUNCHED();                                                                                              \
-        suitor_rounds<LL>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get());                 \
         break;
     switch (L) { BCMGX_MT(4) BCMGX_MT(8) BCMGX_MT(16) BCMGX_MT(32) }
 #undef BCMGX_MT
+    // proposals are latency-bound: half as many lanes per vertex (two entries
+    // each) keeps twice as many proposal chains in flight per wave
+    int LS = L > 4 ? L / 2 : L;
+    switch (LS) {
+    case 4: suitor_rounds<4>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get()); break;
+    case 8: suitor_rounds<8>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get()); break;
+    default: suitor_rounds<16>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get()); break;
+    }
     k_suitor_decode<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Wrow.get(), inv.get(), S.get(), M);
     BCMGX_LAUNCHED();
 }
