@@ -13,49 +13,88 @@
 namespace bcmgx {
 
 // ------------------------------------------------------------ diagonals ---
-__global__ void k_diag_index(int n, const int* __restrict__ row, const int* __restrict__ col, int* __restrict__ dind) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Row-local passes over a CSR matrix with coalesced loads: a block stages the
+// entries of its RS_ROWS consecutive rows in LDS (when they fit), then thread r
+// walks row r in column order.  Rows of a block holding more than RS_CAP
+// entries are walked straight from global memory.
+constexpr int RS_ROWS = 128;
+constexpr int RS_CAP = 4096;
+
+__global__ __launch_bounds__(RS_ROWS) void k_diag_index_s(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                     int* __restrict__ dind) {
+    __shared__ int sc[RS_CAP];
+    int r0 = blockIdx.x * RS_ROWS;
+    int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
+    int base = row[r0], cnt = row[rend] - base;
+    bool staged = cnt <= RS_CAP;
+    if (staged)
+        for (int t = threadIdx.x; t < cnt; t += RS_ROWS) sc[t] = col[base + t];
+    __syncthreads();
+    int i = r0 + threadIdx.x;
     if (i >= n) return;
     int d = -1;
     for (int j = row[i]; j < row[i + 1]; j++)
-        if (col[j] == i) { d = j; break; }
+        if ((staged ? sc[j - base] : col[j]) == i) { d = j; break; }
     dind[i] = d;
 }
+
+__global__ __launch_bounds__(RS_ROWS) void k_abs_row_sum_s(int n, const int* __restrict__ row, const double* __restrict__ val,
+                                                      double* __restrict__ M) {
+    __shared__ double sv[RS_CAP];
+    int r0 = blockIdx.x * RS_ROWS;
+    int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
+    int base = row[r0], cnt = row[rend] - base;
+    bool staged = cnt <= RS_CAP;
+    if (staged)
+        for (int t = threadIdx.x; t < cnt; t += RS_ROWS) sv[t] = val[base + t];
+    __syncthreads();
+    int i = r0 + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    for (int j = row[i]; j < row[i + 1]; j++) t += fabs(staged ? sv[j - base] : val[j]);
+    M[i] = t;
+}
+
 void diag_index(Context* c, int n, const int* row, const int* col, int* dind) {
     if (n <= 0) return;
-    k_diag_index<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, col, dind);
+    k_diag_index_s<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, row, col, dind);
     BCMGX_LAUNCHED();
 }
 
-// CSRm::absoluteRowSum (CSR.cu:1362-1397), column order
-__global__ void k_abs_row_sum(int n, const int* __restrict__ row, const double* __restrict__ val, double* __restrict__ M) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double t = 0.0;
-    for (int j = row[i]; j < row[i + 1]; j++) t += fabs(val[j]);
-    M[i] = t;
-}
+// CSRm::absoluteRowSum (CSR.cu:1362-1397), column order (k_abs_row_sum_s above)
 void abs_row_sum(Context* c, int n, const int* row, const double* val, double* M) {
     if (n <= 0) return;
-    k_abs_row_sum<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, val, M);
+    k_abs_row_sum_s<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, row, val, M);
     BCMGX_LAUNCHED();
 }
 
 // set_l1j (l1jacobi.cu:484-493): mydiag then add_absrowsum_nodiag
-__global__ void k_l1_diag(int n, const int* __restrict__ row, const int* __restrict__ col, const double* __restrict__ val,
-                          double* __restrict__ d) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(RS_ROWS) void k_l1_diag(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                     const double* __restrict__ val, double* __restrict__ d) {
+    __shared__ int sc[RS_CAP];
+    __shared__ double sv[RS_CAP];
+    int r0 = blockIdx.x * RS_ROWS;
+    int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
+    int base = row[r0], cnt = row[rend] - base;
+    bool staged = cnt <= RS_CAP;
+    if (staged)
+        for (int t = threadIdx.x; t < cnt; t += RS_ROWS) {
+            sc[t] = col[base + t];
+            sv[t] = val[base + t];
+        }
+    __syncthreads();
+    int i = r0 + threadIdx.x;
     if (i >= n) return;
     double dg = 0.0, s = 0.0;
     for (int j = row[i]; j < row[i + 1]; j++)
-        if (col[j] == i) dg = val[j];
+        if ((staged ? sc[j - base] : col[j]) == i) dg = staged ? sv[j - base] : val[j];
     for (int j = row[i]; j < row[i + 1]; j++)
-        if (col[j] != i) s += fabs(val[j]);
+        if ((staged ? sc[j - base] : col[j]) != i) s += fabs(staged ? sv[j - base] : val[j]);
     d[i] = dg + s;
 }
 void l1_diag(Context* c, int n, const int* row, const int* col, const double* val, double* d) {
     if (n <= 0) return;
-    k_l1_diag<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, col, val, d);
+    k_l1_diag<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, row, col, val, d);
     BCMGX_LAUNCHED();
 }
 
@@ -161,6 +200,24 @@ __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow
     int lane = (int)threadIdx.x % G;
     if (y >= n) return;
     int s = Wrow[y], e = Wrow[y + 1];
+    if (e - s <= G) {  // uniform over the group: one entry per lane, compared through shuffles
+        int len = e - s;
+        bool va = lane < len;
+        double wa = va ? Wval[s + lane] : 0.0;
+        int ca = va ? Wcol[s + lane] : 0;
+        int r = 0;
+#pragma unroll
+        for (int b = 0; b < G; b++) {
+            double wb = __shfl(wa, b, G);
+            int cb = __shfl(ca, b, G);
+            r += (b < len && (wb < wa || (wb == wa && cb < ca))) ? 1 : 0;
+        }
+        if (va) {
+            rank[s + lane] = r;
+            inv[s + r] = ca;
+        }
+        return;
+    }
     for (int a = s + lane; a < e; a += G) {
         double wa = Wval[a];
         int ca = Wcol[a];
@@ -570,13 +627,21 @@ void transpose_prolongator(Context* c, int n, int nc, const int* agg, const doub
 constexpr int SMALL_P = 64;
 constexpr int MED_P = 2048;  // 2048 x (8 + 8 + 4) B + scan = 41 KiB of LDS
 
-__global__ void k_count_products(int n, const int* __restrict__ arow, const int* __restrict__ acol, const int* __restrict__ brow,
-                                 long long* __restrict__ cnt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __restrict__ arow, const int* __restrict__ acol,
+                                                            const int* __restrict__ brow, long long* __restrict__ cnt) {
+    __shared__ int sc[RS_CAP];
+    int r0 = blockIdx.x * RS_ROWS;
+    int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
+    int base = arow[r0], ne = arow[rend] - base;
+    bool staged = ne <= RS_CAP;
+    if (staged)
+        for (int t = threadIdx.x; t < ne; t += RS_ROWS) sc[t] = acol[base + t];
+    __syncthreads();
+    int i = r0 + threadIdx.x;
     if (i >= n) return;
     long long s = 0;
     for (int j = arow[i]; j < arow[i + 1]; j++) {
-        int k = acol[j];
+        int k = staged ? sc[j - base] : acol[j];
         s += brow[k + 1] - brow[k];
     }
     cnt[i] = s;
@@ -876,7 +941,7 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
         return;
     }
     DBuf<long long> cnt(c, n), poff(c, (size_t)n + 1);
-    k_count_products<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
+    k_count_products<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
     BCMGX_LAUNCHED();
     long long total = scan_exclusive(c, cnt.get(), poff.get(), n);
     size_t tsz = (size_t)std::max<long long>(total, 1);
