@@ -33,6 +33,48 @@ def spmv_fp64_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
+def smoother_bytes(n, nnz, k, fp32):
+    # one l1-Jacobi sweep y = x + (b - A x) / D on k columns: the matrix
+    # (val + col + row pointers) once, x, b and y once per column, D once
+    w = 4 if fp32 else 8
+    return (w + 4) * nnz + 4 * (n + 1) + 3 * w * n * k + w * n
+
+
+def csr_bytes(n, nnz):
+    return 12 * nnz + 4 * (n + 1)
+
+
+def kernel_rooflines(bx, ctx, A, n, nnz, k, reps):
+    """Achieved HBM bandwidth of the smoother (FP64 one column; FP32 k columns,
+    the batched V-cycle's fine sweep) and of the Galerkin SpGEMMs of the first
+    fine matching sweep (A P, then P^T (A P)), HIP-event timed.  SpGEMM bytes are
+    the compulsory ones (both operands and the product once each)."""
+    import numpy as np
+
+    out = {}
+    for name, prec, kk in (("smoother_fp64_k1", 0, 1), (f"smoother_fp32_k{k}", 1, k)):
+        ms = A.time_smoother(precision=prec, k=kk, reps=reps)
+        by = smoother_bytes(n, nnz, kk, prec == 1)
+        out[name] = {"kernel": "l1-Jacobi sweep (JACOBI epilogue of the autotuned SpMV/SpMM)", "bound": "hbm",
+                     "avg_launch_ms": ms, "algorithmic_bytes": by, "achieved": by / (ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    M = A.matching(np.ones(n))
+    agg, pval, nc = bx.make_prolongator(ctx, M, np.ones(n))
+    P = bx.CSR(ctx, n, nc, np.arange(n + 1, dtype=np.int32), agg, pval)
+    R = P.transpose()
+    AP = A.spgemm(P)
+    for name, L, Rm, nl, nzl, nr, nzr in (("spgemm_AP", A, P, n, nnz, n, n), ("spgemm_RAP", R, AP, nc, n, n, AP.nnz)):
+        ms, nzc = L.time_spgemm(Rm, reps=max(1, reps // 4))
+        by = csr_bytes(nl, nzl) + csr_bytes(nr, nzr) + csr_bytes(nl, nzc)
+        out[name] = {"kernel": "deterministic SpGEMM (count, bin, sort-and-sum rows, compact), whole product",
+                     "bound": "hbm", "avg_product_ms": ms, "algorithmic_bytes": by, "nnz_c": nzc,
+                     "achieved": by / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    for h in (AP, R, P):
+        h.close()
+    return out
+
+
 def pmc_traffic(name="spmv_fp64_lap27_128.txt"):
     """HBM bytes per launch of the fine FP64 SpMV from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (copy-kernel calibrated), or None."""
@@ -128,6 +170,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the autograd forward + adjoint timing")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
                     help="only time REPS launches of the FP64 SpMV of the fine matrix (profiling)")
+    ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
+                    help="only time the smoother and the Galerkin SpGEMMs of the fine matrix (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -151,6 +195,11 @@ def main():
         print(json.dumps({"kernel": "spmv fp64", "n": n, "nnz": nnz, "avg_launch_ms": ms, "gbs": gbs,
                           "algorithmic_bytes": spmv_fp64_bytes(n, nnz), "copy_bytes_per_launch": 2 * (1 << 30),
                           "copy_gbs": copy_gbs}))
+        return
+    if args.kernels_only:
+        res = kernel_rooflines(bx, ctx, A, n, nnz, k, args.kernels_only)
+        res["copy_gbs"] = ctx.stream_copy_gbs(1 << 30, 20)  # k_copy16 over 1 GiB: counter calibration
+        print(json.dumps(res))
         return
     Bh = np.asfortranarray(np.random.default_rng(3).normal(size=(n, k)))
     B = bx.DeviceArray(ctx, (n, k))
@@ -211,6 +260,7 @@ def main():
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     spmm32_ms = A.time_spmv(precision=1, k=k, reps=10)
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 20)
+    kernels = kernel_rooflines(bx, ctx, A, n, nnz, k, 20)
 
     adjoint = None if args.no_adjoint else adjoint_timing(ctx)
 
@@ -256,6 +306,7 @@ def main():
             "algorithmic_bytes": spmv_bytes,
             "avg_launch_ms": spmv_ms,
         },
+        "kernel_rooflines": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -90,6 +90,8 @@ def lib():
         "bcmgx_make_prolongator": (C.c_int, [_VP, C.c_int, _I32P, _F64P, _I32P, _F64P, C.POINTER(C.c_int)]),
         "bcmgx_stream_copy_gbs": (C.c_int, [_VP, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
         "bcmgx_time_spmv": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "bcmgx_time_smoother": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "bcmgx_time_spgemm": (C.c_int, [_VP, _VP, _VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "bcmgx_sparse_outer_grad": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -335,6 +337,18 @@ class CSR:
         ms = C.c_double()
         _check(lib().bcmgx_time_spmv(self.ctx.h, self.h, precision, k, lanes, reps, C.byref(ms)), self.ctx)
         return ms.value
+
+    def time_smoother(self, precision=0, k=1, reps=20):
+        """Average device milliseconds of one l1-Jacobi sweep on k columns."""
+        ms = C.c_double()
+        _check(lib().bcmgx_time_smoother(self.ctx.h, self.h, precision, k, reps, C.byref(ms)), self.ctx)
+        return ms.value
+
+    def time_spgemm(self, B, reps=5):
+        """(average device milliseconds of one product self * B, nonzeros of the product)."""
+        ms, nnz = C.c_double(), C.c_int()
+        _check(lib().bcmgx_time_spgemm(self.ctx.h, self.h, B.h, reps, C.byref(ms), C.byref(nnz)), self.ctx)
+        return ms.value, nnz.value
 
     def spgemm(self, B):
         h = C.c_void_p()

@@ -663,6 +663,92 @@ int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int
     API_END(ctx)
 }
 
+int bcmgx_time_smoother(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int reps, double* ms) {
+    if (!ctx || !A || !ms || k < 1 || k > 64 || reps < 1 || (precision != 0 && precision != 1)) return BCMGX_ERR_ARG;
+    API_BEGIN
+    Context* c = ctx->c;
+    BCMGX_HIP(hipSetDevice(c->device));
+    DCsr& M = A->A;
+    BCMGX_CHECK(M.n == M.m && M.n > 0, BCMGX_ERR_ARG, "the smoother needs a nonempty square matrix");
+    bool f32 = precision == 1;
+    if (f32 && !M.valf) {
+        M.valf.reset(c, std::max(M.nnz, 1));
+        convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
+    }
+    int n = M.n;
+    size_t nk = (size_t)n * k;
+    SpmvArgs a;
+    a.n = n; a.row = M.row.get(); a.col = M.col.get(); a.nnz = M.nnz; a.k = k;
+    // the launch shape the V-cycle picks for this operator and batch width
+    if (k == 1) a.lanes = tune_lanes(c, M, f32);
+    else a.kchunk = tune_chunk(c, M, f32, k);
+    DBuf<double> d64(c, n), xd, yd, bd;
+    DBuf<float> df, xf, yf, bf;
+    l1_diag(c, n, M.row.get(), M.col.get(), M.val.get(), d64.get());
+    if (f32) {
+        df.reset(c, n); xf.reset(c, nk); yf.reset(c, nk); bf.reset(c, nk);
+        convert<double, float>(c, d64.get(), df.get(), n);
+        fill<float>(c, xf.get(), (long long)nk, 1.0f);
+        fill<float>(c, bf.get(), (long long)nk, 1.0f);
+        a.val = M.valf.get(); a.x = xf.get(); a.y = yf.get(); a.b = bf.get(); a.D = df.get();
+    } else {
+        xd.reset(c, nk); yd.reset(c, nk); bd.reset(c, nk);
+        fill<double>(c, xd.get(), (long long)nk, 1.0);
+        fill<double>(c, bd.get(), (long long)nk, 1.0);
+        a.val = M.val.get(); a.x = xd.get(); a.y = yd.get(); a.b = bd.get(); a.D = d64.get();
+    }
+    auto launch = [&]() {
+        if (f32) spmv<float, float>(c, SpOp::JACOBI, a);
+        else spmv<double, double>(c, SpOp::JACOBI, a);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    BCMGX_HIP(hipEventCreate(&e0));
+    BCMGX_HIP(hipEventCreate(&e1));
+    BCMGX_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) launch();
+    BCMGX_HIP(hipEventRecord(e1, c->stream));
+    BCMGX_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BCMGX_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return BCMGX_OK;
+    API_END(ctx)
+}
+
+int bcmgx_time_spgemm(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix B, int reps, double* ms, int* nnz_c) {
+    if (!ctx || !A || !B || !ms || reps < 1) return BCMGX_ERR_ARG;
+    API_BEGIN
+    Context* c = ctx->c;
+    BCMGX_HIP(hipSetDevice(c->device));
+    BCMGX_CHECK(A->A.m == B->A.n, BCMGX_ERR_ARG, "spgemm: inner dimensions differ");
+    {
+        DCsr C;
+        spgemm(c, A->A, B->A, C);  // warm-up (and the size of the result)
+        if (nnz_c) *nnz_c = C.nnz;
+    }
+    c->sync();
+    hipEvent_t e0, e1;
+    BCMGX_HIP(hipEventCreate(&e0));
+    BCMGX_HIP(hipEventCreate(&e1));
+    BCMGX_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) {
+        DCsr C;
+        spgemm(c, A->A, B->A, C);
+    }
+    BCMGX_HIP(hipEventRecord(e1, c->stream));
+    BCMGX_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BCMGX_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return BCMGX_OK;
+    API_END(ctx)
+}
+
 int bcmgx_stream_copy_gbs(bcmgx_context ctx, size_t bytes, int reps, double* gbs) {
     if (!ctx || !gbs || bytes < 1024 || reps < 1) return BCMGX_ERR_ARG;
     API_BEGIN

@@ -248,6 +248,17 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
  * k > 1: columns per launch (0 = all k, pipelined kernel; -1 = all k, one-pass kernel;
  * -2 = all k, pipelined kernel with streamed matrix loads). */
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms);
+/* Micro-benchmark of the smoother: one l1-Jacobi sweep y = x + (b - A x) / D
+ * (_jacobi_it_full<0>, l1jacobi.cu:344-379) on k interleaved columns, D = the
+ * l1 diagonal of A, FP64 (precision 0) or FP32 matrix and vectors (precision 1),
+ * with the launch shape the V-cycle picks for this operator and width; `reps`
+ * timed launches after one warm-up, bracketed by HIP events. */
+int bcmgx_time_smoother(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int reps, double* ms);
+/* Micro-benchmark of the SpGEMM C = A*B (the Galerkin products of the setup,
+ * in place of nsparse's spgemm_csrseg_kernel_hash, kernel_spgemm_hash_d.cu:1196-1247):
+ * `reps` whole products (row counting, binning, numeric kernels, compaction)
+ * after one warm-up, bracketed by HIP events; *nnz_c = nonzeros of C. */
+int bcmgx_time_spgemm(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix B, int reps, double* ms, int* nnz_c);
 /* Sustained device-to-device copy bandwidth (GB/s) over `bytes` bytes. */
 int bcmgx_stream_copy_gbs(bcmgx_context ctx, size_t bytes, int reps, double* gbs);
 

@@ -178,6 +178,22 @@ def test_spgemm_medium_and_large_rows(ctx):
     _assert_same_csr(dev(ctx, E).spgemm(dev(ctx, E)), O.spgemm(E, E))
 
 
+def test_kernel_timers(ctx):
+    """The smoother and SpGEMM micro-benchmarks run their kernels and report sane figures."""
+    oa = O.laplacian3d(27, 12, 12, 12)
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    for prec, k in ((0, 1), (1, 1), (1, 8), (0, 5)):
+        assert A.time_smoother(prec, k, 3) > 0
+    w = np.ones(n)
+    agg, pv, nc = O.make_p(O.suitor(oa, w), w)
+    P = dev(ctx, O.Csr.from_arrays(n, nc, np.arange(n + 1), agg, pv))
+    ms, nzc = A.time_spgemm(P, 2)
+    assert ms > 0 and nzc == A.spgemm(P).nnz
+    with pytest.raises(bx.BcmgxError):
+        P.time_spgemm(P, 1)  # inner dimensions differ
+
+
 # --------------------------------------------------------------- hierarchy
 HIER = [
     ("lap7_24", lambda: O.laplacian3d(7, 24, 24, 24), {}),
