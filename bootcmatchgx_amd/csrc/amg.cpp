@@ -110,8 +110,21 @@ int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
     int best = k;
     float bestms = 1e30f;
     // full width through the pipelined (k), one-pass (-1) or pipelined kernel with
-    // streamed matrix loads (-2), then narrower column chunks
+    // streamed matrix loads (-2); the pipelined kernel with rows dealt to the
+    // XCDs in super-groups of 512, 2048 or 8192 rows (-16 - log2 of the
+    // super-group in row groups); then narrower column chunks
     std::vector<int> cands = {k, -1, -2};
+    int cpl = fp32 ? 4 : 2;
+    if (k % cpl == 0 && k >= 2 * cpl && k <= 64) {
+        int KP = 2;
+        while (KP < k / cpl) KP <<= 1;
+        int rpb = 256 / KP;  // rows per row group of the pipelined kernel
+        for (int rows : {512, 2048, 8192}) {
+            int lg = 0;
+            while ((rpb << (lg + 1)) <= rows) lg++;
+            cands.push_back(-16 - lg);
+        }
+    }
     for (int ch = (k + 1) / 2; ch >= 4; ch = (ch + 1) / 2) {
         cands.push_back(ch);
         if (ch == 4) break;
@@ -253,14 +266,20 @@ struct SolveWork {
     bool fused = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipGraph_t cg_graph = nullptr;  // one CG iteration
+    hipGraphExec_t cg_exec = nullptr;
     int lanesA = 0;
     int chunkA = 0;
     int mv_blocks = 0;  // grid of the fused SpMM + dots kernel (0: not used)
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
+        if (cg_graph) (void)hipGraphDestroy(cg_graph);
         exec = nullptr;
         graph = nullptr;
+        cg_exec = nullptr;
+        cg_graph = nullptr;
     }
     ~SolveWork() { drop_graph(); }
 };
@@ -584,10 +603,8 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     fcg_finish_iteration(c, s.fb);
 }
 
-SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k) {
-    BCMGX_CHECK(k >= 1 && k <= 64, BCMGX_ERR_ARG, "nrhs must be in [1, 64]");
-    BCMGX_CHECK(p.solver == BCMGX_SOLVER_FCG, BCMGX_ERR_UNSUPPORTED, "only the FCG solver is on the single-GPU path");
-    double t0 = now_s();
+// the Krylov workspace for k columns of A (kept per k: graphs are bound to it)
+static SolveWork& solve_work(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, int k) {
     int n = A.n;
     SolveWork* sp = nullptr;
     for (auto& w : prec->solves)
@@ -614,7 +631,15 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         s.lanesA = p.autotune ? tune_lanes(c, A, false) : default_lanes(A.nnz, A.n);
         s.chunkA = p.autotune ? tune_chunk(c, A, false, k) : k;
     }
-    SolveWork& s = *sp;
+    return *sp;
+}
+
+SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k) {
+    BCMGX_CHECK(k >= 1 && k <= 64, BCMGX_ERR_ARG, "nrhs must be in [1, 64]");
+    BCMGX_CHECK(p.solver == BCMGX_SOLVER_FCG, BCMGX_ERR_STATE, "fcg_solve called for another solver");
+    double t0 = now_s();
+    int n = A.n;
+    SolveWork& s = solve_work(c, A, prec, p, k);
     s.fb.rtol = p.rtol;
     s.fb.itnlim = p.itnlim;
     prec->ensure_work(k);
@@ -691,6 +716,102 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         res.exit_res = std::max(res.exit_res, st[j].l2);
         res.rel_res = std::max(res.rel_res, rel);
         if (st[j].iter >= p.itnlim) res.retv = -1;  // FCG.cu:160-162
+        if (!std::isfinite(st[j].l2)) res.retv = BCMGX_ERR_NUMERIC;
+    }
+    res.del0 = st[0].delta0;
+    res.seconds = now_s() - t0;
+    return res;
+}
+
+// cg_hs (src/solver/cghs/CG_HS.cu:24-172) on k interleaved columns: the FCG
+// workspace, preconditioners and vector kernels, every scalar on the device
+// (per-column masks), the general iteration replayed from a hipGraph.
+SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k) {
+    BCMGX_CHECK(k >= 1 && k <= 64, BCMGX_ERR_ARG, "nrhs must be in [1, 64]");
+    BCMGX_CHECK(p.solver == BCMGX_SOLVER_CGHS, BCMGX_ERR_STATE, "cghs_solve called for another solver");
+    double t0 = now_s();
+    int n = A.n;
+    SolveWork& s = solve_work(c, A, prec, p, k);
+    s.fb.rtol = p.rtol;
+    s.fb.itnlim = p.itnlim;
+    prec->ensure_work(k);
+    long long nk = (long long)n * k;
+    copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
+    copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
+    fill<int>(c, s.flags.get(), 2, 0);
+    double* U = s.v.get();  // preconditioned residual
+    double* S = s.w.get();  // A p
+    double* P = s.d.get();  // direction
+    double* R = s.r.get();
+    const int* stop = s.flags.get();
+    auto spmm = [&](const double* xin, double* yout, SpOp op, const int* st) {
+        SpmvArgs a;
+        a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        a.x = xin; a.y = yout; a.b = s.B.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA; a.stop = st;
+        spmv<double, double>(c, op, a);
+    };
+    spmm(s.X.get(), R, SpOp::RES, nullptr);  // r = b - A x  (CG_HS.cu:53-57)
+    prec->apply(R, U, k, true, nullptr);    // u = B r, p = u  (CG_HS.cu:59-68)
+    copy_d2d(c, P, U, sizeof(double) * nk);
+    cg_init(c, s.fb, R, p.stop_criterion);
+    if (p.itnlim >= 1) {
+        // iteration 0 (CG_HS.cu:110-135): no preconditioner application, rho = u.r
+        spmm(P, S, SpOp::AX, stop);
+        cg_alpha(c, s.fb, P, S, U, R, true);
+        cg_update_finish(c, s.fb, P, S, s.X.get(), R);
+        auto iteration = [&]() {  // CG_HS.cu:87-135
+            prec->apply(R, U, k, true, stop);
+            cg_beta_direction(c, s.fb, R, U, P);
+            spmm(P, S, SpOp::AX, stop);
+            cg_alpha(c, s.fb, P, S, U, R, false);
+            cg_update_finish(c, s.fb, P, S, s.X.get(), R);
+        };
+        bool use_graph = p.use_graph != 0;
+        if (use_graph && !s.cg_exec) {
+            BCMGX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                iteration();
+            } catch (...) {
+                hipGraph_t g;
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            BCMGX_HIP(hipStreamEndCapture(c->stream, &s.cg_graph));
+            BCMGX_HIP(hipGraphInstantiate(&s.cg_exec, s.cg_graph, nullptr, nullptr, 0));
+        }
+        int check = std::max(1, p.check_every);
+        int launched = 1;
+        int* hflag = (int*)c->host_scratch(sizeof(int) * 2);
+        while (launched < p.itnlim) {
+            for (int j = 0; j < check && launched < p.itnlim; j++, launched++) {
+                if (use_graph) BCMGX_HIP(hipGraphLaunch(s.cg_exec, c->stream));
+                else iteration();
+            }
+            BCMGX_HIP(hipMemcpyAsync(hflag, s.flags.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+            c->sync();
+            if (hflag[0]) break;
+        }
+    }
+    copy_d2d(c, x, s.X.get(), sizeof(double) * nk);
+    std::vector<FcgState> st(k);
+    c->d2h(st.data(), s.st.get(), k);
+    SolveResult res;
+    prec->last_niter.assign(k, 0);
+    prec->last_exit.assign(k, 0);
+    prec->last_rel.assign(k, 0);
+    for (int j = 0; j < k; j++) {
+        bool conv = st[j].pad0 != 0;
+        // out->niter = iter + 1 + excnlim (CG_HS.cu:165-171): iter + 1 when it converged, else itnlim
+        int niter = p.itnlim < 1 ? 0 : (conv ? st[j].iter + 1 : st[j].iter);
+        double rel = st[j].delta0 > 0 ? st[j].l2 / st[j].delta0 : 0.0;
+        prec->last_niter[j] = niter;
+        prec->last_exit[j] = st[j].l2;
+        prec->last_rel[j] = rel;
+        res.niter = std::max(res.niter, niter);
+        res.exit_res = std::max(res.exit_res, st[j].l2);
+        res.rel_res = std::max(res.rel_res, rel);
+        if (!conv) res.retv = -1;
         if (!std::isfinite(st[j].l2)) res.retv = BCMGX_ERR_NUMERIC;
     }
     res.del0 = st[0].delta0;

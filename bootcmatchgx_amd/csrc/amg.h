@@ -83,6 +83,8 @@ struct SolveResult {
 };
 // FCG on k interleaved columns (device pointers b, x; x in/out)
 SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
+// CG (Hestenes-Stiefel) on k interleaved columns, same conventions
+SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
 
 // autotune: fastest lanes per row for SpMV with A (k = 1)
 int tune_lanes(Context* c, const DCsr& A, bool fp32);

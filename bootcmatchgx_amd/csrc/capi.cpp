@@ -400,8 +400,9 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     if (!ctx || !A || !p || !b || !x || nrhs < 1 || nrhs > 64) return BCMGX_ERR_ARG;
     if (A->ctx != ctx || (prec && prec->ctx != ctx)) return BCMGX_ERR_STATE;
     API_BEGIN
-    // solve() dispatches on the solver type (src/solver/solve.cu:19-40); only FCG is on this path
-    BCMGX_CHECK(p->solver == BCMGX_SOLVER_FCG, BCMGX_ERR_UNSUPPORTED, "only the FCG solver is supported");
+    // solve() dispatches on the solver type (src/solver/solve.cu:19-40): FCG and CGHS here
+    BCMGX_CHECK(p->solver == BCMGX_SOLVER_FCG || p->solver == BCMGX_SOLVER_CGHS, BCMGX_ERR_UNSUPPORTED,
+                "only the FCG and CGHS solvers are supported");
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
     int n = A->A.n;
@@ -420,7 +421,8 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     DBuf<double> B(c, nk), X(c, nk), tmp;
     stage_in(c, b, ld, mem, n, nrhs, B.get(), tmp);
     stage_in(c, x, ld, mem, n, nrhs, X.get(), tmp);
-    SolveResult r = fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
+    SolveResult r = p->solver == BCMGX_SOLVER_CGHS ? cghs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
+                                                   : fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
     stage_out(c, X.get(), x, ld, mem, n, nrhs, tmp);
     c->sync();
     if (out) {
@@ -436,7 +438,7 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     if (prec && own == nullptr) {
         // results kept on the handle for bcmgx_solve_columns
     }
-    return r.retv == BCMGX_ERR_NUMERIC ? record(ctx, BCMGX_ERR_NUMERIC, "non-finite residual norm (FCG.cu:158)") : BCMGX_OK;
+    return r.retv == BCMGX_ERR_NUMERIC ? record(ctx, BCMGX_ERR_NUMERIC, "non-finite residual norm") : BCMGX_OK;
     API_END(ctx)
 }
 
@@ -595,7 +597,7 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
 
 int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
                double beta, int op, int kchunk, int mem) {
-    if (!ctx || !A || !x || !y || op < 0 || op > 2 || k < 1 || k > 64 || kchunk < -2) return BCMGX_ERR_ARG;
+    if (!ctx || !A || !x || !y || op < 0 || op > 2 || k < 1 || k > 64 || (kchunk < -2 && (kchunk > -16 || kchunk < -28))) return BCMGX_ERR_ARG;
     API_BEGIN
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
@@ -630,7 +632,7 @@ int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int
         convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
     }
     if (k == 1) a.lanes = lanes != 0 ? lanes : tune_lanes(c, M, precision == 1);
-    else a.kchunk = lanes >= -2 ? lanes : 0;
+    else a.kchunk = (lanes >= -2 || (lanes <= -16 && lanes >= -28)) ? lanes : 0;
     DBuf<double> xd, yd;
     DBuf<float> xf, yf;
     if (precision == 1) {

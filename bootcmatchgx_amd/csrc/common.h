@@ -192,6 +192,18 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     int per = G / X, rem = G % X;
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
+// The other order: super-blocks of sb consecutive logical blocks dealt
+// round-robin to the XCDs, so all XCDs work on one band of rows at a time
+// (sb <= 0: xcd_remap).  Bijective on [0, G); the blocks of an incomplete last
+// round keep their own index.
+__device__ __forceinline__ int xcd_order(int b, int G, int sb) {
+    if (sb <= 0) return xcd_remap(b, G);
+    const int X = 8;
+    int full = G / (X * sb) * (X * sb);
+    if (b >= full) return b;
+    int xcd = b % X, local = b / X;
+    return ((local / sb) * X + xcd) * sb + local % sb;
+}
 template <typename T>
 __device__ __forceinline__ T ldg_nt(const T* p) { return __builtin_nontemporal_load(p); }
 }  // namespace bcmgx

@@ -12,6 +12,8 @@
 // column has its own scalars and stop flag.
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace bcmgx {
 
 // partial-sum blocks of the streaming FCG kernels: a function of n only, so
@@ -286,6 +288,190 @@ __global__ __launch_bounds__(WG) void k_finish(const double* __restrict__ partia
 
 void fcg_finish_iteration(Context* c, const FcgBufs& f) {
     k_finish<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.rtol, f.itnlim, f.all_done, f.all_done + 1);
+    BCMGX_LAUNCHED();
+}
+
+// ---- CG (Hestenes-Stiefel), cg_hs (src/solver/cghs/CG_HS.cu:24-172) ---------
+// FcgState fields as used by CG: alpha = rho = r.u of the current iteration,
+// delta = rho of the previous one, theta = beta, gamma = p.s, theta2 = alpha
+// (the step), pad0 = 1 once the column converged.
+
+// delta0 = ||r|| (stop_criterion 1) or 1 (CG_HS.cu:74-79); a zero initial
+// residual stops the column at once (the reference would divide 0 by 0)
+__global__ __launch_bounds__(WG) void k_cg_init(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                                int stopcrit, int* all_done, int* ndone) {
+    __shared__ double s[1][WG];
+    double tot[1];
+    int c = blockIdx.x;
+    sum_partials<1>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        double l2 = sqrt(tot[0]);
+        S.l2 = l2;
+        S.delta0 = stopcrit == 1 ? l2 : 1.0;
+        S.iter = 0;
+        S.done = 0;
+        S.pad0 = 0;
+        S.delta = 0.0;
+        if (l2 == 0.0) {
+            S.done = 1;
+            S.pad0 = 1;
+            if (atomicAdd(ndone, 1) + 1 == k) *all_done = 1;
+        }
+    }
+}
+
+void cg_init(Context* c, const FcgBufs& f, const double* r, int stopcrit) {
+    int KP = kpad(f.k);
+    k_dots<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, r, nullptr, nullptr, nullptr, nullptr, f.partials, nullptr);
+    BCMGX_LAUNCHED();
+    k_cg_init<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, stopcrit, f.all_done, f.all_done + 1);
+    BCMGX_LAUNCHED();
+}
+
+// rho = r.u, beta = rho / rho_prev  (CG_HS.cu:89-101)
+__global__ __launch_bounds__(WG) void k_cg_beta(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                                const int* __restrict__ all_done) {
+    if (*all_done) return;
+    __shared__ double s[1][WG];
+    double tot[1];
+    int c = blockIdx.x;
+    sum_partials<1>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        if (S.done) return;
+        S.alpha = tot[0];
+        S.theta = S.alpha / S.delta;
+    }
+}
+
+// p = 1 u + beta p  (my_axpby, basic.cu:114-130; CG_HS.cu:103-107)
+__global__ __launch_bounds__(WG) void k_cg_direction(int n, int k, int KP, const FcgState* __restrict__ st,
+                                                     const double* __restrict__ u, double* __restrict__ p,
+                                                     const int* __restrict__ all_done) {
+    if (*all_done) return;
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    if (c >= k || st[c].done) return;
+    double be = st[c].theta;
+    for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+        long long o = i * k + c;
+        p[o] = 1.0 * u[o] + be * p[o];
+    }
+}
+
+void cg_beta_direction(Context* c, const FcgBufs& f, const double* r, const double* u, double* p) {
+    int KP = kpad(f.k);
+    k_dots<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, u, nullptr, nullptr, nullptr, nullptr, f.partials, f.all_done);
+    BCMGX_LAUNCHED();
+    k_cg_beta<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.all_done);
+    BCMGX_LAUNCHED();
+    k_cg_direction<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, u, p, f.all_done);
+    BCMGX_LAUNCHED();
+}
+
+// (first iteration: rho = u.r) gamma = p.s, alpha = rho / gamma; rho_prev = rho  (CG_HS.cu:112-129)
+template <int NV>
+__global__ __launch_bounds__(WG) void k_cg_alpha(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                                 const int* __restrict__ all_done) {
+    if (*all_done) return;
+    __shared__ double s[NV][WG];
+    double tot[NV];
+    int c = blockIdx.x;
+    sum_partials<NV>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        if (S.done) return;
+        if (NV == 2) S.alpha = tot[1];
+        S.gamma = tot[0];
+        S.theta2 = S.alpha / S.gamma;
+        S.delta = S.alpha;
+    }
+}
+
+void cg_alpha(Context* c, const FcgBufs& f, const double* p, const double* s, const double* u, const double* r, bool first) {
+    int KP = kpad(f.k);
+    if (first) {
+        k_dots<2><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, p, s, u, r, nullptr, nullptr, f.partials, f.all_done);
+        BCMGX_LAUNCHED();
+        k_cg_alpha<2><<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.all_done);
+    } else {
+        k_dots<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, p, s, nullptr, nullptr, nullptr, nullptr, f.partials, f.all_done);
+        BCMGX_LAUNCHED();
+        k_cg_alpha<1><<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.all_done);
+    }
+    BCMGX_LAUNCHED();
+}
+
+// x = alpha p + 1 x; r = -alpha s + 1 r (CG_HS.cu:131-132), with the ||r||^2 partials
+__global__ __launch_bounds__(WG) void k_cg_update(int n, int k, int KP, const FcgState* __restrict__ st,
+                                                  const double* __restrict__ p, const double* __restrict__ sv,
+                                                  double* __restrict__ x, double* __restrict__ r,
+                                                  double* __restrict__ partials, const int* __restrict__ all_done) {
+    if (*all_done) return;
+    __shared__ double s[WG];
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    double acc = 0.0;
+    if (c < k) {
+        const FcgState S = st[c];
+        double al = S.theta2;
+        for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+            long long o = i * k + c;
+            double rr;
+            if (!S.done) {
+                x[o] = al * p[o] + 1.0 * x[o];
+                rr = -al * sv[o] + 1.0 * r[o];
+                r[o] = rr;
+            } else {
+                rr = r[o];
+            }
+            acc += rr * rr;
+        }
+    }
+    s[tid] = acc;
+    __syncthreads();
+    for (int stp = WG / 2; stp >= KP; stp >>= 1) {
+        if (tid < stp) s[tid] += s[tid + stp];
+        __syncthreads();
+    }
+    if (tid < k) partials[((long long)blockIdx.x * k + tid) * 3] = s[tid];
+}
+
+// l2 = ||r||; converged when l2 < rtol delta0, else the iteration count
+// advances and itnlim stops the column (CG_HS.cu:134-163)
+__global__ __launch_bounds__(WG) void k_cg_finish(const double* __restrict__ partials, int nblk, int k, FcgState* st,
+                                                  double rtol, int itnlim, int* all_done, int* ndone) {
+    if (*all_done) return;
+    __shared__ double s[1][WG];
+    double tot[1];
+    int c = blockIdx.x;
+    sum_partials<1>(partials, nblk, k, c, tot, s);
+    if (threadIdx.x == 0) {
+        FcgState& S = st[c];
+        if (S.done) return;
+        S.l2 = sqrt(tot[0]);
+        bool stop = false;
+        if (S.l2 < rtol * S.delta0) {
+            S.pad0 = 1;
+            stop = true;
+        } else if (++S.iter >= itnlim) {
+            stop = true;
+        }
+        if (stop) {
+            S.done = 1;
+            if (atomicAdd(ndone, 1) + 1 == k) *all_done = 1;
+        }
+    }
+}
+
+void cg_update_finish(Context* c, const FcgBufs& f, const double* p, const double* s, double* x, double* r) {
+    int KP = kpad(f.k);
+    k_cg_update<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p, s, x, r, f.partials, f.all_done);
+    BCMGX_LAUNCHED();
+    k_cg_finish<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.rtol, f.itnlim, f.all_done, f.all_done + 1);
     BCMGX_LAUNCHED();
 }
 

@@ -43,7 +43,9 @@ struct SpmvArgs {
     int k = 1;                  // number of vector columns
     int lanes = 0;              // lanes per row for k == 1 (0 = heuristic, < 0 = CSR-stream kernel)
     int kchunk = 0;             // k > 1: columns per launch (0 = all k, pipelined kernel; -1 = all k, one-pass
-                                // kernel; -2 = all k, pipelined kernel with streamed matrix loads)
+                                // kernel; -2 = all k, pipelined kernel with streamed matrix loads;
+                                // -16-s = all k, pipelined kernel with rows dealt to the XCDs in
+                                // super-groups of 2^s row groups)
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
     double* y64 = nullptr;      // JACOBI with V = float: write the result widened to FP64 here (y is scratch)
 };
@@ -113,6 +115,16 @@ void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, doub
                 const FcgEntry& e = FcgEntry());
 // ||r|| -> convergence test and iteration count (FCG.cu:146-156)
 void fcg_finish_iteration(Context* c, const FcgBufs& f);
+
+// ---- CG (Hestenes-Stiefel) kernels (fcg.hip), CG_HS.cu:24-172 ----------------
+// ||r|| -> delta0 (||r|| for stop_criterion 1, else 1), iteration count 0
+void cg_init(Context* c, const FcgBufs& f, const double* r, int stopcrit);
+// rho = r.u, beta = rho / rho_prev, p = u + beta p
+void cg_beta_direction(Context* c, const FcgBufs& f, const double* r, const double* u, double* p);
+// gamma = p.s (first iteration also rho = u.r), alpha = rho / gamma
+void cg_alpha(Context* c, const FcgBufs& f, const double* p, const double* s, const double* u, const double* r, bool first);
+// x += alpha p, r -= alpha s, ||r|| -> convergence test, iteration count, itnlim
+void cg_update_finish(Context* c, const FcgBufs& f, const double* p, const double* s, double* x, double* r);
 
 // ---- setup kernels (setup.hip) -------------------------------------------------
 // position of the first diagonal entry of each row (-1 if none)

@@ -188,9 +188,13 @@ struct SpmmExtra {
 };
 
 // Software-pipelined persistent k-vector SpMM, CPL consecutive columns per
-// lane (16-byte gathers), KP lanes per row.  The rows are cut into 8
-// contiguous segments, one per XCD (hardware dispatch puts block b on XCD
-// b % 8), and the blocks of an XCD stride through its segment.  Each KP-lane
+// lane (16-byte gathers), KP lanes per row.  Rows go to XCDs (hardware
+// dispatch puts block b on XCD b % 8) in one of two ways: sg == 0 cuts them
+// into 8 contiguous segments, one per XCD; sg > 0 deals super-groups of sg
+// row groups round-robin to the XCDs, so that all XCDs sweep the same band of
+// rows at once and a row's far neighbours (a stencil's next plane) fall on
+// the XCD, and into the L2, that loaded them as its own rows' x a short while
+// before.  The blocks of an XCD stride through its rows in order.  Each KP-lane
 // group walks its own sequence of rows with a three-stage pipeline: the row
 // bounds of row i+2 and the first M*KP (col, val) entries of row i+1 are in
 // flight while row i's x gathers (16 per lane) are.  Every column's sum runs
@@ -204,7 +208,7 @@ template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, boo
 __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
                                               const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
                                               const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
-                                              const int* __restrict__ stop, SpmmExtra ex) {
+                                              const int* __restrict__ stop, SpmmExtra ex, int sg) {
     typedef V VV __attribute__((ext_vector_type(CPL)));
     typedef double VD __attribute__((ext_vector_type(CPL)));
     if (stop && *stop) return;
@@ -225,10 +229,16 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
     T* sv = sval + g * E;
     const V* xc = x + cl * CPL;
     VD da = (VD)0.0, db = (VD)0.0, dc = (VD)0.0;
+    // the t-th row group of this XCD, -1 past its last (increasing in t)
+    auto group_of = [&](int q) -> int {
+        if (sg <= 0) return g0 + q < g1 ? g0 + q : -1;
+        long long gq = ((long long)(q / sg) * X + xcd) * sg + q % sg;
+        return gq < ngroups ? (int)gq : -1;
+    };
     // row grp*RPB + g (-1 past the end) and its bounds
     auto bounds = [&](int grp, int& ri, int& rs, int& re) {
         int pos = grp * RPB + g;
-        if (grp < g1 && pos < n) {
+        if (grp >= 0 && pos < n) {
             ri = pos;
             rs = row[ri];
             re = row[ri + 1];
@@ -267,16 +277,17 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
             }
         }
     };
-    int grp = g0 + local;  // uniform over the block
+    int it = local;  // uniform over the block
+    int grp = group_of(it);
     int r0, s0, e0, r1, s1, e1;
     bounds(grp, r0, s0, e0);
-    bounds(grp + nloc, r1, s1, e1);
+    bounds(group_of(it + nloc), r1, s1, e1);
     int pc[M];
     T pv[M];
     chunk(s0, e0, pc, pv);
-    while (grp < g1) {
+    while (grp >= 0) {
         int r2, s2, e2;
-        bounds(grp + 2 * nloc, r2, s2, e2);
+        bounds(group_of(it + 2 * nloc), r2, s2, e2);
 #pragma unroll
         for (int m = 0; m < M; m++) {
             sc[m * KP + cl] = pc[m];
@@ -322,7 +333,8 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
                 else *yo = ((-t + *(const VV*)(b + o)) / D[ri]) + *(const VV*)(x + o);
             }
         }
-        grp += nloc;
+        it += nloc;
+        grp = group_of(it);
         r0 = r1; s0 = s1; e0 = e1;
         r1 = r2; s1 = s2; e1 = e2;
 #pragma unroll
@@ -369,6 +381,11 @@ static int persistent_grid(Context* c, K kernel) {
     return std::max(8, g / 8 * 8);
 }
 
+// super-group size (row groups) of the XCD-interleaved row order that a column
+// choice asks for: kchunk <= -16 is the pipelined kernel with super-groups of
+// 2^(-16 - kchunk) groups; other choices keep the per-XCD segments (0)
+static int kchunk_sg(int kchunk) { return kchunk <= -16 && kchunk >= -28 ? 1 << (-16 - kchunk) : 0; }
+
 template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT>
 static void launch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int grid) {
     auto kern = k_spmmp<T, V, KP, CPL, M, OP, MODE, NT>;
@@ -378,7 +395,7 @@ static void launch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int
         grid = cached;
     }
     kern<<<grid, WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, (const T*)a.val, (const V*)a.x, (V*)a.y, (const V*)a.b,
-                                     (const V*)a.D, (V)a.alpha, (V)a.beta, a.stop, ex);
+                                     (const V*)a.D, (V)a.alpha, (V)a.beta, a.stop, ex, kchunk_sg(a.kchunk));
     BCMGX_LAUNCHED();
 }
 
@@ -447,14 +464,14 @@ template <typename T, typename V, int TPR, SpOp OP>
 __global__ __launch_bounds__(WG) void k_spmv_stream(int n, const int* __restrict__ row, const int* __restrict__ col,
                                                     const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
                                                     const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
-                                                    const int* __restrict__ stop) {
+                                                    const int* __restrict__ stop, int sb) {
     if (stop && *stop) return;
     constexpr int RB = WG / TPR;
     constexpr int CAP = RB * 32 < 4096 ? RB * 32 : 4096;
     constexpr int U = 8;
     __shared__ V prod[CAP];
     __shared__ int rp[RB + 1];
-    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int blk = xcd_order(blockIdx.x, gridDim.x, sb);
     int r0 = blk * RB;
     int tid = threadIdx.x;
     // the block's nonzero range from two uniform (scalar) loads; the per-row
@@ -525,7 +542,7 @@ __global__ __launch_bounds__(WG) void k_spmv_stream2(int n, long long nnz, const
                                                      const int* __restrict__ col, const T* __restrict__ val,
                                                      const V* __restrict__ x, V* __restrict__ y, const V* __restrict__ b,
                                                      const V* __restrict__ D, V alpha, V beta,
-                                                     const int* __restrict__ stop) {
+                                                     const int* __restrict__ stop, int sb) {
     typedef T T2 __attribute__((ext_vector_type(2)));
     typedef int I2 __attribute__((ext_vector_type(2)));
     if (stop && *stop) return;
@@ -534,7 +551,7 @@ __global__ __launch_bounds__(WG) void k_spmv_stream2(int n, long long nnz, const
     constexpr int UP = 4;  // pairs per thread per pass
     __shared__ V prod[CAP];
     __shared__ int rp[RB + 1];
-    int blk = xcd_remap(blockIdx.x, gridDim.x);
+    int blk = xcd_order(blockIdx.x, gridDim.x, sb);
     int r0 = blk * RB;
     int tid = threadIdx.x;
     int rend = r0 + RB < n ? r0 + RB : n;
@@ -633,6 +650,16 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
     const V* b = (const V*)a.b;
     const V* D = (const V*)a.D;
     V al = (V)a.alpha, be = (V)a.beta;
+    // L < 0: CSR-stream kernels, L = -(1000 s + P + T): T lanes per row
+    // reduction, P = 200 for the (val, col) pair stream, s = 0 for per-XCD
+    // row segments or s >= 1 for XCD-interleaved super-blocks of 2^(s-1) blocks
+    int sb = 0;
+    if (L < 0) {
+        int s = -L / 1000;
+        if (s > 13) throw Error(BCMGX_ERR_ARG, "stream super-block code out of range");
+        sb = s > 0 ? 1 << (s - 1) : 0;
+        L = -(-L % 1000);
+    }
     if (L <= -200) {  // CSR-stream variant streaming (val, col) pairs, TPR = -L - 200 lanes per row
         bool aligned = (((uintptr_t)a.val) & (2 * sizeof(T) - 1)) == 0 && (((uintptr_t)a.col) & 7) == 0;
         if (!aligned) L += 200;  // the single-entry stream kernel serves unaligned arrays
@@ -641,7 +668,7 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
 #define BCMGX_S2(TT)                                                                                           \
     case TT:                                                                                                   \
         k_spmv_stream2<T, V, TT, OP><<<grid_for(a.n, WG / TT), WG, 0, c->stream>>>(a.n, a.nnz, a.row, a.col, val, x, \
-                                                                                  y, b, D, al, be, a.stop);    \
+                                                                                  y, b, D, al, be, a.stop, sb); \
         break;
         switch (-L - 200) {
             BCMGX_S2(1) BCMGX_S2(2) BCMGX_S2(4) BCMGX_S2(8) BCMGX_S2(16)
@@ -655,7 +682,7 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
 #define BCMGX_S(TT)                                                                                                 \
     case TT:                                                                                                        \
         k_spmv_stream<T, V, TT, OP><<<grid_for(a.n, WG / TT), WG, 0, c->stream>>>(a.n, a.row, a.col, val, x, y, b, D, \
-                                                                                 al, be, a.stop);                   \
+                                                                                 al, be, a.stop, sb);               \
         break;
         switch (-L) {
             BCMGX_S(1) BCMGX_S(2) BCMGX_S(4) BCMGX_S(8) BCMGX_S(16)

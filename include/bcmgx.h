@@ -225,7 +225,9 @@ int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, do
  * counterpart in the reference, which multiplies one vector at a time: it is
  * the multi-vector kernel of the batched solve.  kchunk: columns per launch
  * (0 = all k through the pipelined persistent kernel, -1 = all k through the
- * one-pass kernel, -2 = pipelined with streamed matrix loads).  Each column is the sequential row sum of that column
+ * one-pass kernel, -2 = pipelined with streamed matrix loads, -16-s (s in
+ * 0..12) = pipelined with rows dealt to the XCDs in super-groups of 2^s row
+ * groups instead of one contiguous segment per XCD).  Each column is the sequential row sum of that column
  * (the oracle's order) bit for bit. */
 int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
                double beta, int op, int kchunk, int mem);
@@ -246,7 +248,8 @@ int bcmgx_make_prolongator(bcmgx_context ctx, int n, const int* M, const double*
  * warm-up, bracketed by HIP events.  *ms = average milliseconds per launch.
  * `lanes`: k == 1: lanes per row (0 = autotuned, -T = CSR-stream kernel, -200-T = paired-load CSR-stream);
  * k > 1: columns per launch (0 = all k, pipelined kernel; -1 = all k, one-pass kernel;
- * -2 = all k, pipelined kernel with streamed matrix loads). */
+ * -2 = all k, pipelined kernel with streamed matrix loads; -16-s = all k, pipelined
+ * kernel with XCD-interleaved super-groups of 2^s row groups). */
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms);
 /* Micro-benchmark of the smoother: one l1-Jacobi sweep y = x + (b - A x) / D
  * (_jacobi_it_full<0>, l1jacobi.cu:344-379) on k interleaved columns, D = the

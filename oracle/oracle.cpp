@@ -1084,6 +1084,62 @@ int or_fcg(const ocsr* Ao, const oprec* pr, const bcmgx_params* pp, const double
     return iter >= p.itnlim ? -1 : 0;
 }
 
+// cg_hs (CG_HS.cu:24-172): preconditioned CG, Hestenes-Stiefel recurrences.
+// my_axpby(x, y, a, b) is y = a x + b y (basic.cu:114-130).
+int or_cghs(const ocsr* Ao, const oprec* pr, const bcmgx_params* pp, const double* b, double* x,
+            int* niter_out, double* exit_res, double* del0_out)
+{
+    const Csr& A = Ao->A;
+    const bcmgx_params& p = *pp;
+    int n = A.n;
+    bool prec = p.preconditioner != BCMGX_PREC_NONE;
+    std::vector<double> s(n, 0.0), pv(n), r(b, b + n), w(n), u(n, 0.0);
+    spmv(A, x, w.data(), 1.0, 0.0, 1);
+    for (int i = 0; i < n; i++) r[i] = -1.0 * w[i] + 1.0 * r[i];  // CG_HS.cu:53-57
+    if (prec) {
+        prec_apply_impl(pr, r.data(), u.data());  // u = 0 on entry (CG_HS.cu:60-64)
+        pv = u;
+    } else {
+        pv = r;
+    }
+    double delta0 = 1.0;
+    if (p.stop_criterion == 1) delta0 = std::sqrt(dot(r.data(), r.data(), n));
+    double rho = 0.0, l2 = 0.0;
+    int iter, excnlim = -1, retval = -1;
+    for (iter = 0; iter < p.itnlim; iter++) {
+        if (iter > 0) {  // CG_HS.cu:87-108
+            double rho_new;
+            if (prec) {
+                std::fill(u.begin(), u.end(), 0.0);
+                prec_apply_impl(pr, r.data(), u.data());
+                rho_new = dot(r.data(), u.data(), n);
+            } else {
+                rho_new = dot(r.data(), r.data(), n);
+            }
+            double beta = rho_new / rho;
+            const std::vector<double>& z = prec ? u : r;
+            for (int i = 0; i < n; i++) pv[i] = 1.0 * z[i] + beta * pv[i];
+        }
+        spmv(A, pv.data(), s.data(), 1.0, 0.0, 1);  // CG_HS.cu:110
+        rho = prec ? dot(u.data(), r.data(), n) : dot(r.data(), r.data(), n);
+        double ps = dot(pv.data(), s.data(), n);
+        double alpha = rho / ps;
+        for (int i = 0; i < n; i++) x[i] = alpha * pv[i] + 1.0 * x[i];
+        for (int i = 0; i < n; i++) r[i] = -alpha * s[i] + 1.0 * r[i];
+        l2 = std::sqrt(dot(r.data(), r.data(), n));
+        if (p.dispnorm > 1) printf("%d) r norm: %.10lf\n", iter, l2);
+        if (l2 < p.rtol * delta0) {
+            excnlim = 0;
+            retval = 0;
+            break;
+        }
+    }
+    *niter_out = iter + 1 + excnlim;
+    *exit_res = l2;
+    *del0_out = delta0;
+    return retval;
+}
+
 int or_num_threads(void)
 {
 #ifdef _OPENMP

@@ -65,6 +65,9 @@ void or_prec_apply(const oprec* pr, const double* r, double* u);
 /* FCG (FCG.cu:27-175).  x in/out.  Returns retv. */
 int or_fcg(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
            int* niter, double* exit_res, double* del0);
+// cg_hs (CG_HS.cu:24-172); same outputs as or_fcg (niter counted as the reference does)
+int or_cghs(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
+            int* niter, double* exit_res, double* del0);
 
 int or_num_threads(void);
 

@@ -63,7 +63,8 @@ def oparams(p):
 
 # ------------------------------------------------------------------- SpMV
 @pytest.mark.parametrize("name", ["lap7_12", "lap27_10", "rand"])
-@pytest.mark.parametrize("lanes", [0, 1, 4, 16, 64, -1, -2, -4, -16, -201, -204, -208, -216])
+@pytest.mark.parametrize("lanes", [0, 1, 4, 16, 64, -1, -2, -4, -16, -201, -204, -208, -216,
+                                   -1004, -2204, -3002, -1216, -13204])
 def test_spmv_vs_oracle(ctx, name, lanes):
     oa = CASES[name]()
     A = dev(ctx, oa)
@@ -87,7 +88,8 @@ SPMM_CASES = {
 
 @pytest.mark.parametrize("name", list(SPMM_CASES))
 @pytest.mark.parametrize("k,kchunk", [(2, 0), (3, 0), (4, -1), (8, 0), (8, -2), (13, 4), (16, -1), (32, 0), (32, -1),
-                                     (32, -2), (32, 8), (64, 0), (64, -1), (64, -2), (64, 16)])
+                                     (32, -2), (32, 8), (64, 0), (64, -1), (64, -2), (64, 16),
+                                     (8, -16), (16, -18), (32, -17), (32, -22), (64, -20)])
 def test_spmm_columns_bit_exact(ctx, name, k, kchunk):
     """Every column of the multi-vector kernel equals the oracle's sequential
     row sum (same order, no contraction) bit for bit."""
@@ -421,12 +423,57 @@ def test_sparse_outer_grad(ctx):
     assert np.allclose(g, -np.sum(lam[rows] * x[c], axis=1), rtol=1e-14, atol=1e-14)
 
 
-def test_non_fcg_solver_is_rejected(ctx):
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI", "BCMG", "AFSAI"])
+@pytest.mark.parametrize("name", ["lap7_12", "lap27_10", "rand"])
+def test_cghs_matches_oracle(ctx, name, prec):
+    """CG (CG_HS.cu): iteration count within 1 of the oracle, solution to 1e-9."""
+    oa = CASES[name]()
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    p = params(prec, solver="CGHS", rtol=1e-10, itnlim=3000)
+    b = np.random.default_rng(11).normal(size=n)
+    xo, io = O.cghs(oa, O.Prec(oa, oparams(p)), oparams(p), b)
+    P = bx.prec_setup(ctx, A, p)
+    x, out = bx.solve(ctx, A, b, None, p, P)
+    P.finalize()
+    assert out["retv"] == io["retv"] == 0
+    assert abs(out["niter"] - io["niter"]) <= 1
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
+
+
+def test_cghs_batched_columns_and_stop_rules(ctx):
+    """k columns in one batched CG equal one-at-a-time solves; itnlim and the absolute test."""
+    oa = O.laplacian3d(27, 14, 13, 12)
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    p = params("BCMG", solver="CGHS", rtol=1e-9)
+    P = bx.prec_setup(ctx, A, p)
+    B = np.random.default_rng(5).normal(size=(n, 4))
+    B[:, 2] *= 1e-3  # scaled: the relative test gives the same count, the absolute one does not
+    X, out = bx.solve(ctx, A, B, None, p, P)
+    OP = O.Prec(oa, oparams(p))
+    for j in range(4):
+        xj, oj = bx.solve(ctx, A, B[:, j].copy(), None, p, P)
+        # the dot products of a k-column batch sum in another (fixed) order than one column's
+        assert np.linalg.norm(X[:, j] - xj) / np.linalg.norm(xj) < 1e-10
+        assert abs(out["niter_cols"][j] - oj["niter"]) <= 1
+        xo, io = O.cghs(oa, OP, oparams(p), B[:, j])
+        assert abs(oj["niter"] - io["niter"]) <= 1
+    q = p.copy().set("rtol", 1e-30).set("itnlim", 3)
+    _, o3 = bx.solve(ctx, A, B[:, 0].copy(), None, q, P)
+    assert o3["retv"] == -1 and o3["niter"] == 3
+    q = p.copy().set("rtol", 1e-6).set("stop_criterion", 0)
+    x0, o0 = bx.solve(ctx, A, B[:, 1].copy(), None, q, P)
+    assert o0["retv"] == 0 and o0["del0"] == 1.0 and o0["exit_res"] < 1e-6
+    P.finalize()
+
+
+def test_unsupported_solver_is_rejected(ctx):
     oa = CASES["lap7_12"]()
     A = dev(ctx, oa)
     p = params("BCMG")
     P = bx.prec_setup(ctx, A, p)
-    p.set("solver", "CGHS")
+    p.set("solver", "CGS")
     with pytest.raises(bx.BcmgxError) as e:
         bx.solve(ctx, A, np.ones(A.n), None, p, P)
     assert e.value.status == -6 and "FCG" in str(e.value)

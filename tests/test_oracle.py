@@ -279,6 +279,104 @@ def test_fp32_hierarchy_oracle_converges():
     assert i32["retv"] == 0 and abs(i32["niter"] - i64["niter"]) <= 1
 
 
+# ------------------------------------------------------------ CG (CG_HS.cu)
+def cghs_params(prec, **kw):
+    p = fcg_params(prec, **kw)
+    p.solver = abi.SOLVER["CGHS"]
+    return p
+
+
+@pytest.mark.parametrize("pts", [7, 27])
+@pytest.mark.parametrize("prec", ["BCMG", "AFSAI", "L1_JACOBI", "NONE"])
+def test_cghs_matches_reference_fixture(golden, pts, prec):
+    """The fixtures' A x = 1 solutions hold for any converged solver (launchAndCompare.mk, 1e-5)."""
+    A = O.laplacian3d(pts, 10, 10, 10)
+    p = cghs_params(prec)
+    x, info = O.cghs(A, O.Prec(A, p), p, np.ones(1000))
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_1x1x1_sol.txt"))
+    assert info["retv"] == 0
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+def textbook_pcg(S, b, apply_m, rtol, itnlim):
+    """Preconditioned CG as printed in Saad, Iterative Methods, Alg. 9.1 (independent of the oracle's code)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = apply_m(r)
+    p = z.copy()
+    rz = r @ z
+    tol = rtol * np.linalg.norm(r)
+    for it in range(itnlim):
+        Ap = S @ p
+        a = rz / (p @ Ap)
+        x += a * p
+        r -= a * Ap
+        if np.linalg.norm(r) < tol:
+            return x, it + 1
+        z = apply_m(r)
+        rz, rz_old = r @ z, rz
+        p = z + (rz / rz_old) * p
+    return x, itnlim
+
+
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI"])
+def test_cghs_matches_textbook_pcg(prec):
+    """Iteration count and solution against Saad's PCG, with the l1-Jacobi preconditioner
+    (4 sweeps from zero, divisor a_ii + sum_{j!=i} |a_ij|, l1jacobi.cu:484-523) restated in numpy."""
+    oa = O.laplacian3d(27, 12, 11, 10)
+    S = oa.to_scipy()
+    n = S.shape[0]
+    b = np.random.default_rng(7).normal(size=n)
+    d = S.diagonal()
+    m = d + (abs(S).sum(axis=1).A1 - abs(d))
+
+    def l1j(r):
+        u = np.zeros_like(r)
+        for _ in range(4):
+            u = u + (r - S @ u) / m
+        return u
+
+    apply_m = (lambda r: r.copy()) if prec == "NONE" else l1j
+    p = cghs_params(prec, rtol=1e-10, itnlim=2000, l1jacsweeps=4)
+    x, info = O.cghs(oa, O.Prec(oa, p), p, b)
+    xt, it = textbook_pcg(S, b, apply_m, 1e-10, 2000)
+    assert info["retv"] == 0 and abs(info["niter"] - it) <= 1
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) < 1e-8
+    if prec == "NONE":  # scipy's CG stops on ||r|| <= rtol ||b|| from x0 = 0: the same test up to the tie
+        import scipy.sparse.linalg as spl
+        count = [0]
+        spl.cg(S, b, rtol=1e-10, maxiter=2000, callback=lambda _: count.__setitem__(0, count[0] + 1))
+        assert abs(count[0] - info["niter"]) <= 1
+
+
+def test_cghs_finite_termination_on_distinct_eigenvalues():
+    """CG on an SPD matrix with m distinct eigenvalues converges in m steps (exact arithmetic)."""
+    ev = np.array([1.0, 2.0, 5.0, 7.0, 11.0])
+    n = 400
+    diag = ev[np.arange(n) % 5]
+    oa = O.Csr.from_arrays(n, n, np.arange(n + 1), np.arange(n), diag)
+    b = np.random.default_rng(2).uniform(0.5, 1.5, n)
+    p = cghs_params("NONE", rtol=1e-12, itnlim=50)
+    x, info = O.cghs(oa, O.Prec(oa, p), p, b)
+    assert info["retv"] == 0 and info["niter"] in (5, 6)
+    assert np.allclose(x, b / diag, rtol=1e-10)
+
+
+def test_cghs_stop_rules():
+    A = O.laplacian3d(7, 12, 12, 12)
+    b = np.ones(A.dims()[0])
+    # itnlim: retv -1 and niter = itnlim (iter + 1 + excnlim, CG_HS.cu:165-171)
+    p = cghs_params("L1_JACOBI", rtol=1e-30, itnlim=3)
+    _, info = O.cghs(A, O.Prec(A, p), p, b)
+    assert info["retv"] == -1 and info["niter"] == 3
+    # stop_criterion 0: absolute residual, delta0 = 1 (CG_HS.cu:74-79)
+    p = cghs_params("L1_JACOBI", rtol=1e-6, stop_criterion=0)
+    x, info = O.cghs(A, O.Prec(A, p), p, b)
+    assert info["retv"] == 0 and info["del0"] == 1.0
+    true_res = np.linalg.norm(b - O.spmv(A, x))
+    assert info["exit_res"] < 1e-6 and abs(true_res - info["exit_res"]) <= 1e-9
+
+
 # -------------------------------------------------------------------- AFSAI
 def test_afsai_factor_invariants():
     A = O.laplacian3d(7, 8, 7, 6)
