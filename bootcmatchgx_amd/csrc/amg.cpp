@@ -21,10 +21,6 @@
 
 namespace bcmgx {
 
-static double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 // ------------------------------------------------------------ autotune ----
 int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     if (A.n < 65536) return default_lanes(A.nnz, A.n);

@@ -77,6 +77,10 @@ struct Prec {
 
 std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& p);
 
+inline double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 struct SolveResult {
     int retv = 0, niter = 0;
     double exit_res = 0, rel_res = 0, del0 = 0, seconds = 0;
@@ -85,6 +89,8 @@ struct SolveResult {
 SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
 // CG (Hestenes-Stiefel) on k interleaved columns, same conventions
 SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
+// s-step CG (Chronopoulos-Gear), s = p.sstep, the columns one after another
+SolveResult cgs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
 
 // autotune: fastest lanes per row for SpMV with A (k = 1)
 int tune_lanes(Context* c, const DCsr& A, bool fp32);

@@ -401,8 +401,8 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     if (A->ctx != ctx || (prec && prec->ctx != ctx)) return BCMGX_ERR_STATE;
     API_BEGIN
     // solve() dispatches on the solver type (src/solver/solve.cu:19-40): FCG and CGHS here
-    BCMGX_CHECK(p->solver == BCMGX_SOLVER_FCG || p->solver == BCMGX_SOLVER_CGHS, BCMGX_ERR_UNSUPPORTED,
-                "only the FCG and CGHS solvers are supported");
+    BCMGX_CHECK(p->solver == BCMGX_SOLVER_FCG || p->solver == BCMGX_SOLVER_CGHS || p->solver == BCMGX_SOLVER_CGS,
+                BCMGX_ERR_UNSUPPORTED, "only the FCG, CGHS and CGS solvers are supported");
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
     int n = A->A.n;
@@ -421,8 +421,9 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     DBuf<double> B(c, nk), X(c, nk), tmp;
     stage_in(c, b, ld, mem, n, nrhs, B.get(), tmp);
     stage_in(c, x, ld, mem, n, nrhs, X.get(), tmp);
-    SolveResult r = p->solver == BCMGX_SOLVER_CGHS ? cghs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
-                                                   : fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
+    SolveResult r = p->solver == BCMGX_SOLVER_CGHS  ? cghs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
+                    : p->solver == BCMGX_SOLVER_CGS ? cgs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
+                                                    : fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
     stage_out(c, X.get(), x, ld, mem, n, nrhs, tmp);
     c->sync();
     if (out) {

@@ -126,6 +126,20 @@ void cg_alpha(Context* c, const FcgBufs& f, const double* p, const double* s, co
 // x += alpha p, r -= alpha s, ||r|| -> convergence test, iteration count, itnlim
 void cg_update_finish(Context* c, const FcgBufs& f, const double* p, const double* s, double* x, double* r);
 
+// ---- block-vector kernels of the s-step solvers (krylov.hip) -------------------
+// out[j] = A[:, j] . B[:, j] (B column j at B + j*ldb; ldb = 0: one vector), fixed-order sums;
+// partials: nblk * m doubles
+void block_dots(Context* c, int n, int m, const double* A, const double* B, long long ldb, double* partials, int nblk,
+                double* out);
+// C = C + A B (A n x s, B s x s device), E = E + f (C al)   (cdmmv)
+void block_dmmv(Context* c, int n, int s, const double* A, const double* B, double* C, const double* al, double* E, double f);
+// C = C + (A B)   (cdmm)
+void block_dmm(Context* c, int n, int s, const double* A, const double* B, double* C);
+// D = C + f (A b)   (cdmv2v)
+void block_dmv2v(Context* c, int n, int s, const double* A, const double* bv, const double* C, double f, double* D);
+// dst[i * ds] = src[i * ss] for i < n
+void strided_copy(Context* c, double* dst, int ds, const double* src, int ss, int n);
+
 // ---- setup kernels (setup.hip) -------------------------------------------------
 // position of the first diagonal entry of each row (-1 if none)
 void diag_index(Context* c, int n, const int* row, const int* col, int* dind);

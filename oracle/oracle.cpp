@@ -1140,6 +1140,198 @@ int or_cghs(const ocsr* Ao, const oprec* pr, const bcmgx_params* pp, const doubl
     return retval;
 }
 
+// ---- s-step CG (Chronopoulos-Gear), CGs.cu:43-236 -------------------------
+// Blocks of vectors are column-major n x m arrays.
+
+// mpk (krylov_base/mpk.cu:10-78): without a preconditioner sP[:,0] = x and
+// sP[:,i+1] = A sP[:,i] (i < s); with one, sP[:,0] = M x and 2s-1 alternating
+// products, A into the upper half and M into the lower half of the columns.
+static void sstep_mpk(const Csr& A, const oprec* pr, bool prec, const double* x, int s, double* sP)
+{
+    int n = A.n;
+    if (!prec) {
+        std::copy(x, x + n, sP);
+        for (int i = 0; i < s; i++) spmv(A, sP + (size_t)i * n, sP + (size_t)(i + 1) * n, 1.0, 0.0, 1);
+        return;
+    }
+    std::fill(sP, sP + n, 0.0);
+    prec_apply_impl(pr, x, sP);
+    int pin = 0, pout = s;
+    for (int i = 0; i < 2 * s - 1; i++) {
+        double* xi = sP + (size_t)pin * n;
+        double* yo = sP + (size_t)pout * n;
+        if (i % 2 == 0) {
+            spmv(A, xi, yo, 1.0, 0.0, 1);
+        } else {
+            std::fill(yo, yo + n, 0.0);
+            prec_apply_impl(pr, xi, yo);
+        }
+        int t = pin;
+        pin = pout;
+        pout = t + 1;
+    }
+}
+
+// computevm (krylov_base/computevm.cu:6-31)
+static void sstep_vm(const double* sP, int s, const double* r, int n, bool prec, std::vector<double>& vm)
+{
+    vm.assign(2 * s, 0.0);
+    if (!prec) {
+        for (int j = 0; j <= s; j++) vm[j] = dot(sP + (size_t)j * n, r, n);
+        for (int j = 0; j < s - 1; j++) vm[s + 1 + j] = dot(sP + (size_t)(1 + j) * n, sP + (size_t)s * n, n);
+    } else {
+        for (int j = 0; j < s; j++) vm[j] = dot(sP + (size_t)j * n, r, n);
+        for (int j = 0; j < s; j++) vm[s + j] = dot(sP + (size_t)j * n, sP + (size_t)(2 * s - 1) * n, n);
+    }
+}
+
+// Cholesky factorisation A = L L^T in place (column-major, lower part), then
+// nrhs solves of L L^T X = B (B column-major s x nrhs); >0: not positive definite
+// (the LAPACK dpotrf/dpotrs pair of LBsolve/LBsolvem, LBfunctions.c:23-63)
+static int chol_solve(std::vector<double>& Wm, double* B, int s, int nrhs)
+{
+    for (int j = 0; j < s; j++) {
+        double d = Wm[j + j * s];
+        for (int k = 0; k < j; k++) d -= Wm[j + k * s] * Wm[j + k * s];
+        if (!(d > 0.0)) return j + 1;
+        d = std::sqrt(d);
+        Wm[j + j * s] = d;
+        for (int i = j + 1; i < s; i++) {
+            double t = Wm[i + j * s];
+            for (int k = 0; k < j; k++) t -= Wm[i + k * s] * Wm[j + k * s];
+            Wm[i + j * s] = t / d;
+        }
+    }
+    for (int c = 0; c < nrhs; c++) {
+        double* b = B + (size_t)c * s;
+        for (int i = 0; i < s; i++) {
+            double t = b[i];
+            for (int k = 0; k < i; k++) t -= Wm[i + k * s] * b[k];
+            b[i] = t / Wm[i + i * s];
+        }
+        for (int i = s - 1; i >= 0; i--) {
+            double t = b[i];
+            for (int k = i + 1; k < s; k++) t -= Wm[k + i * s] * b[k];
+            b[i] = t / Wm[i + i * s];
+        }
+    }
+    return 0;
+}
+
+// scalarWorkMO (op/scalarWorkMO.cu:8-85, the LAPACK build): alpha (s) and
+// beta (s x s, column-major) from the moments vm, W kept between iterations
+static int sstep_scalar(const std::vector<double>& vm, std::vector<double>& W, std::vector<double>& alpha,
+                        std::vector<double>& beta, int s, int iter)
+{
+    if (iter == 0) {
+        for (int i = 0; i < s; i++)
+            for (int j = 0; j < s; j++) W[i * s + j] = vm[i + j + 1];
+        std::vector<double> Wc(W);
+        std::copy(vm.begin(), vm.begin() + s, alpha.begin());
+        return chol_solve(Wc, alpha.data(), s, 1);
+    }
+    std::vector<double> rhs1(2 * s - 1, 0.0), b1(s * s);
+    for (int i = 0; i < s; i++) {
+        rhs1[s - 1 + i] = vm[i];
+        for (int j = 0; j < i; j++) rhs1[s - 1 + i] += rhs1[s - 1 + j] * alpha[s - i + j - 1];
+        rhs1[s - 1 + i] = -rhs1[s - 1 + i] / alpha[s - 1];
+    }
+    for (int i = 0; i < s; i++)
+        for (int j = 0; j < s; j++) b1[i + j * s] = beta[i + j * s] = -rhs1[i + j];
+    int info = chol_solve(W, beta.data(), s, s);  // W of the previous iteration
+    if (info) return info;
+    for (int i = 0; i < s; i++)
+        for (int j = 0; j < s; j++) W[i * s + j] = vm[i + j + 1];
+    for (int i = 0; i < s; i++)  // W = W - b1^T beta (LBdgemm, LBfunctions.c:66-69)
+        for (int j = 0; j < s; j++) {
+            double t = 0.0;
+            for (int k = 0; k < s; k++) t += b1[k + i * s] * beta[k + j * s];
+            W[i + j * s] = -1.0 * t + 1.0 * W[i + j * s];
+        }
+    std::vector<double> Wc(W);
+    std::copy(vm.begin(), vm.begin() + s, alpha.begin());
+    return chol_solve(Wc, alpha.data(), s, 1);
+}
+
+// mydmmv (op/basic.cu:387-457): C = C + A B (A n x s, B s x s), then
+// E = E + f (C alpha), the row sum over the columns in order
+static void sstep_dmmv(const double* Am, const std::vector<double>& B, double* Cm, const std::vector<double>& al,
+                       double* E, double f, int n, int s)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < s; j++) {
+            double ro = Cm[(size_t)j * n + i];
+            for (int k = 0; k < s; k++) ro += Am[(size_t)k * n + i] * B[k + s * j];
+            Cm[(size_t)j * n + i] = ro;
+            double t = ro * al[j];
+            acc = j == 0 ? t : acc + t;
+        }
+        E[i] += f * acc;
+    }
+}
+
+// cgsstep (solver/cgs/CGs.cu:43-236)
+int or_cgs(const ocsr* Ao, const oprec* pr, const bcmgx_params* pp, const double* b, double* x,
+           int* niter_out, double* exit_res, double* del0_out)
+{
+    const Csr& A = Ao->A;
+    const bcmgx_params& p = *pp;
+    int n = A.n, s = std::max(1, p.sstep);
+    bool prec = p.preconditioner != BCMGX_PREC_NONE;
+    bool cp = !prec && p.ru_res;
+    std::vector<double> r(b, b + n), w(n);
+    auto residual = [&]() {
+        spmv(A, x, w.data(), 1.0, 0.0, 1);
+        for (int i = 0; i < n; i++) r[i] = -1.0 * w[i] + 1.0 * b[i];
+    };
+    residual();
+    std::vector<double> sP1((size_t)n * 2 * s, 0.0), sP2((size_t)n * 2 * s), sCP1, sCP2;
+    if (cp) {
+        sCP1.assign((size_t)n * s, 0.0);
+        sCP2.assign((size_t)n * s, 0.0);
+    }
+    std::vector<double> vm, alpha(s, 0.0), beta(s * s, 0.0), W(s * s, 0.0);
+    double delta0 = 1.0, l2 = 0.0;
+    if (p.stop_criterion == 1) delta0 = std::sqrt(dot(r.data(), r.data(), n));
+    int iter, info = 0, excnlim = -1, retval = -1;
+    for (iter = 0; iter < p.itnlim; iter++) {
+        if (iter > 0 && info == 0 && p.ru_res && p.rec_res_int > 0 && iter % p.rec_res_int == 0) residual();
+        // even iterations build the basis in sP2 and update with sP1, odd ones the reverse
+        std::vector<double>& cur = iter % 2 == 0 ? sP2 : sP1;
+        std::vector<double>& old = iter % 2 == 0 ? sP1 : sP2;
+        std::vector<double>& ccur = iter % 2 == 0 ? sCP2 : sCP1;
+        std::vector<double>& cold = iter % 2 == 0 ? sCP1 : sCP2;
+        sstep_mpk(A, pr, prec, r.data(), s, cur.data());
+        sstep_vm(cur.data(), s, r.data(), n, prec, vm);
+        info = sstep_scalar(vm, W, alpha, beta, s, iter);
+        if (info != 0) {
+            retval = info;
+            break;
+        }
+        if (cp) std::copy(cur.begin() + n, cur.begin() + (size_t)n * (s + 1), ccur.begin());
+        sstep_dmmv(old.data(), beta, cur.data(), alpha, x, 1.0, n, s);
+        if (p.ru_res) {
+            if (prec) sstep_dmmv(old.data() + (size_t)s * n, beta, cur.data() + (size_t)s * n, alpha, r.data(), -1.0, n, s);
+            else sstep_dmmv(cold.data(), beta, ccur.data(), alpha, r.data(), -1.0, n, s);
+        } else {
+            residual();
+        }
+        l2 = std::sqrt(dot(r.data(), r.data(), n));
+        if (p.dispnorm > 1) printf("%d) r norm: %.10lf\n", iter, l2);
+        if (l2 < p.rtol * delta0) {
+            excnlim = 0;
+            retval = 0;
+            break;
+        }
+    }
+    *niter_out = iter + 1 + excnlim;
+    *exit_res = l2;
+    *del0_out = delta0;
+    return retval;
+}
+
 int or_num_threads(void)
 {
 #ifdef _OPENMP

@@ -68,6 +68,9 @@ int or_fcg(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* 
 // cg_hs (CG_HS.cu:24-172); same outputs as or_fcg (niter counted as the reference does)
 int or_cghs(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
             int* niter, double* exit_res, double* del0);
+// cgsstep (CGs.cu:43-236), s = the sstep setting; same outputs
+int or_cgs(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
+           int* niter, double* exit_res, double* del0);
 
 int or_num_threads(void);
 

@@ -468,12 +468,52 @@ def test_cghs_batched_columns_and_stop_rules(ctx):
     P.finalize()
 
 
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI", "BCMG", "AFSAI"])
+@pytest.mark.parametrize("s", [1, 2, 3])
+def test_sstep_cg_matches_oracle(ctx, prec, s):
+    """s-step CG (CGs.cu): outer iteration count within 1 of the oracle, solution to 1e-8."""
+    oa = CASES["lap27_10"]() if prec != "NONE" else CASES["lap7_12"]()
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    p = params(prec, solver="CGS", sstep=s, rtol=1e-9, itnlim=3000)
+    b = np.random.default_rng(13).normal(size=n)
+    xo, io = O.cgs(oa, O.Prec(oa, oparams(p)), oparams(p), b)
+    P = bx.prec_setup(ctx, A, p)
+    x, out = bx.solve(ctx, A, b, None, p, P)
+    P.finalize()
+    assert out["retv"] == io["retv"] == 0
+    assert abs(out["niter"] - io["niter"]) <= 1
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_sstep_cg_options_columns_and_breakdown(ctx):
+    oa = O.laplacian3d(7, 11, 10, 9)
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    B = np.random.default_rng(17).normal(size=(n, 3))
+    for kw in (dict(ru_res=0), dict(rec_res_int=3), dict()):
+        p = params("L1_JACOBI", solver="CGS", sstep=2, rtol=1e-9, **kw)
+        P = bx.prec_setup(ctx, A, p)
+        X, out = bx.solve(ctx, A, B, None, p, P)
+        for j in range(3):
+            xo, io = O.cgs(oa, O.Prec(oa, oparams(p)), oparams(p), B[:, j])
+            assert abs(out["niter_cols"][j] - io["niter"]) <= 1
+            assert np.linalg.norm(X[:, j] - xo) / np.linalg.norm(xo) < 1e-8
+        P.finalize()
+    # a negative definite operator: the s x s Cholesky fails and retv > 0 (scalarWorkMO)
+    N = O.Csr.from_arrays(50, 50, np.arange(51), np.arange(50), -np.linspace(1, 2, 50))
+    Nd = dev(ctx, N)
+    p = params("NONE", solver="CGS", sstep=1)
+    _, out = bx.solve(ctx, Nd, np.ones(50), None, p, None)
+    assert out["retv"] > 0
+
+
 def test_unsupported_solver_is_rejected(ctx):
     oa = CASES["lap7_12"]()
     A = dev(ctx, oa)
     p = params("BCMG")
     P = bx.prec_setup(ctx, A, p)
-    p.set("solver", "CGS")
+    p.set("solver", "PIPELINED_CGS")
     with pytest.raises(bx.BcmgxError) as e:
         bx.solve(ctx, A, np.ones(A.n), None, p, P)
     assert e.value.status == -6 and "FCG" in str(e.value)

@@ -377,6 +377,76 @@ def test_cghs_stop_rules():
     assert info["exit_res"] < 1e-6 and abs(true_res - info["exit_res"]) <= 1e-9
 
 
+# ------------------------------------------------- s-step CG (CGs.cu, mpk.cu)
+def cgs_params(prec, s, **kw):
+    p = fcg_params(prec, **kw)
+    p.solver = abi.SOLVER["CGS"]
+    p.sstep = s
+    return p
+
+
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI", "BCMG", "AFSAI"])
+def test_sstep_one_is_pcg(prec):
+    """With s = 1 the s-step recurrences are PCG's: same counts and iterates as CGHS."""
+    A = O.laplacian3d(7, 12, 12, 11)
+    b = np.random.default_rng(3).normal(size=A.dims()[0])
+    pc = cghs_params(prec, rtol=1e-10, itnlim=3000)
+    ps = cgs_params(prec, 1, rtol=1e-10, itnlim=3000)
+    xc, ic = O.cghs(A, O.Prec(A, pc), pc, b)
+    xs, is_ = O.cgs(A, O.Prec(A, ps), ps, b)
+    assert is_["retv"] == 0 and abs(is_["niter"] - ic["niter"]) <= 1
+    assert np.linalg.norm(xs - xc) / np.linalg.norm(xc) < 1e-8
+
+
+@pytest.mark.parametrize("prec,s", [("NONE", 2), ("NONE", 3), ("L1_JACOBI", 2), ("L1_JACOBI", 3), ("BCMG", 2)])
+def test_sstep_tracks_cg_every_s_steps(prec, s):
+    """s-step CG reproduces CG's iterate every s steps (exact arithmetic): about niter_CG / s
+    outer iterations, and the same solution, against Saad's PCG restated in numpy."""
+    oa = O.laplacian3d(7, 10, 9, 8)
+    S = oa.to_scipy()
+    n = S.shape[0]
+    b = np.random.default_rng(4).normal(size=n)
+    p = cgs_params(prec, s, rtol=1e-8, itnlim=3000)
+    x, info = O.cgs(oa, O.Prec(oa, p), p, b)
+    assert info["retv"] == 0
+    P = O.Prec(oa, cghs_params(prec))
+    _, it = textbook_pcg(S, b, lambda r: r.copy() if prec == "NONE" else P.apply(r), 1e-8, 3000)
+    assert abs(info["niter"] - int(np.ceil(it / s))) <= 1
+    assert np.linalg.norm(b - S @ x) <= 1e-8 * np.linalg.norm(b) * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("pts", [7, 27])
+def test_sstep_matches_reference_fixture(golden, pts):
+    A = O.laplacian3d(pts, 10, 10, 10)
+    p = cgs_params("BCMG", 2)
+    x, info = O.cgs(A, O.Prec(A, p), p, np.ones(1000))
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_1x1x1_sol.txt"))
+    assert info["retv"] == 0
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+def test_sstep_residual_options_and_breakdown():
+    A = O.laplacian3d(7, 9, 9, 9)
+    n = A.dims()[0]
+    b = np.random.default_rng(5).normal(size=n)
+    base = cgs_params("L1_JACOBI", 2, rtol=1e-8)
+    xb, ib = O.cgs(A, O.Prec(A, base), base, b)
+    # ru_res = 0: the residual is recomputed as b - A x after each update (CGs.cu:148-152)
+    p0 = cgs_params("L1_JACOBI", 2, rtol=1e-8, ru_res=0)
+    x0, i0 = O.cgs(A, O.Prec(A, p0), p0, b)
+    # rec_res_int: periodic recomputation (CGs.cu:121-131)
+    p1 = cgs_params("L1_JACOBI", 2, rtol=1e-8, rec_res_int=3)
+    x1, i1 = O.cgs(A, O.Prec(A, p1), p1, b)
+    for x, i in ((x0, i0), (x1, i1)):
+        assert i["retv"] == 0 and abs(i["niter"] - ib["niter"]) <= 1
+        assert np.linalg.norm(x - xb) / np.linalg.norm(xb) < 1e-6
+    # a negative definite operator: the s x s Cholesky fails, retv > 0 (scalarWorkMO)
+    N = O.Csr.from_arrays(50, 50, np.arange(51), np.arange(50), -np.linspace(1, 2, 50))
+    pn = cgs_params("NONE", 1)
+    _, inn = O.cgs(N, O.Prec(N, pn), pn, np.ones(50))
+    assert inn["retv"] > 0
+
+
 # -------------------------------------------------------------------- AFSAI
 def test_afsai_factor_invariants():
     A = O.laplacian3d(7, 8, 7, 6)
