@@ -89,7 +89,8 @@ struct SolveResult {
 SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
 // CG (Hestenes-Stiefel) on k interleaved columns, same conventions
 SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
-// s-step CG (Chronopoulos-Gear), s = p.sstep, the columns one after another
+// s-step CG (Chronopoulos-Gear; p.solver CGS or CGS_CUBLAS) or its pipelined
+// form (PIPELINED_CGS), s = p.sstep, the columns one after another
 SolveResult cgs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
 
 // autotune: fastest lanes per row for SpMV with A (k = 1)

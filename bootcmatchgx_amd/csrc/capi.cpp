@@ -400,9 +400,11 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     if (!ctx || !A || !p || !b || !x || nrhs < 1 || nrhs > 64) return BCMGX_ERR_ARG;
     if (A->ctx != ctx || (prec && prec->ctx != ctx)) return BCMGX_ERR_STATE;
     API_BEGIN
-    // solve() dispatches on the solver type (src/solver/solve.cu:19-40): FCG and CGHS here
-    BCMGX_CHECK(p->solver == BCMGX_SOLVER_FCG || p->solver == BCMGX_SOLVER_CGHS || p->solver == BCMGX_SOLVER_CGS,
-                BCMGX_ERR_UNSUPPORTED, "only the FCG, CGHS and CGS solvers are supported");
+    // solve() dispatches on the solver type (src/solver/solve.cu:19-40).  CGS_CUBLAS
+    // (cgs_cublas/CGscublas.cu) is the s-step method with cuBLAS doing the block
+    // products: the same s-step kernels serve it.
+    BCMGX_CHECK(p->solver >= BCMGX_SOLVER_CGHS && p->solver <= BCMGX_SOLVER_CGS_CUBLAS, BCMGX_ERR_UNSUPPORTED,
+                "Unhandled value for enum class SolverType");
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
     int n = A->A.n;
@@ -422,7 +424,7 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     stage_in(c, b, ld, mem, n, nrhs, B.get(), tmp);
     stage_in(c, x, ld, mem, n, nrhs, X.get(), tmp);
     SolveResult r = p->solver == BCMGX_SOLVER_CGHS  ? cghs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
-                    : p->solver == BCMGX_SOLVER_CGS ? cgs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
+                    : p->solver != BCMGX_SOLVER_FCG ? cgs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
                                                     : fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
     stage_out(c, X.get(), x, ld, mem, n, nrhs, tmp);
     c->sync();

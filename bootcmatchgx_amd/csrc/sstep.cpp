@@ -1,5 +1,6 @@
-// sstep.cpp -- s-step CG (Chronopoulos-Gear, solver/cgs/CGs.cu:43-236) on the
-// device kernels.  The Krylov basis (mpk, solver/krylov_base/mpk.cu:10-78) is
+// sstep.cpp -- s-step CG (Chronopoulos-Gear, solver/cgs/CGs.cu:43-236) and its
+// pipelined form (solver/pipelined_cgs/pipelinedCGs.cu:56-300) on the device
+// kernels.  The Krylov basis (mpk, solver/krylov_base/mpk.cu:10-78) is
 // built with the library's SpMV and preconditioner applications into
 // column-major n x 2s blocks, the 2s moments (computevm,
 // krylov_base/computevm.cu) are reduced in a fixed order and read back, the
@@ -210,11 +211,103 @@ ColumnResult cgs_column(SstepRun& R, const bcmgx_params& p, const double* b, dou
     return res;
 }
 
+// pipecomputevm (pipelinedCGs.cu:7-27), read back
+void pipe_moments(SstepRun& R, const double* UR, const double* UQ, const double* r, std::vector<double>& vm) {
+    Context* c = R.c;
+    int n = R.n, s = R.s;
+    double* out = R.dvm.get();
+    const double* q = UQ + (size_t)(s - 1) * n;
+    block_dots(c, n, s, UR, r, 0, R.partials.get(), R.nblk, out);
+    if (R.precon) {
+        block_dots(c, n, s, UR, q, 0, R.partials.get(), R.nblk, out + s);
+    } else {
+        block_dots(c, n, 1, q, r, 0, R.partials.get(), R.nblk, out + s);
+        block_dots(c, n, s - 1, UQ, q, 0, R.partials.get(), R.nblk, out + s + 1);
+    }
+    vm.assign(2 * s, 0.0);
+    c->d2h(vm.data(), out, 2 * s);
+}
+
+// pipelinedcgsstep (pipelinedCGs.cu:56-300) on one right-hand side.  URm/UQm
+// hold s+1 blocks of n x s (block j at j*s*n); sets 1 and 2 swap roles
+// between even and odd iterations.
+ColumnResult pipecgs_column(SstepRun& R, const bcmgx_params& p, const double* b, double* x) {
+    Context* c = R.c;
+    int n = R.n, s = R.s;
+    size_t bs = (size_t)n * s;
+    DBuf<double> r(c, n), sP2(c, (size_t)n * 2 * s), sT(c, (size_t)n * 2 * s), ams(c, n);
+    DBuf<double> UR1m(c, bs * (s + 1)), UR2m(c, bs * (s + 1)), UQ1m(c, bs * (s + 1)), UQ2m(c, bs * (s + 1));
+    DBuf<double> UR1(c, bs), UR2(c, bs), UQ1(c, bs), UQ2(c, bs);
+    for (DBuf<double>* z : {&UR1m, &UR2m, &UQ1m, &UQ2m}) fill<double>(c, z->get(), (long long)bs * (s + 1), 0.0);
+    for (DBuf<double>* z : {&UR1, &UR2, &UQ1, &UQ2}) fill<double>(c, z->get(), (long long)bs, 0.0);
+    R.ax(x, r.get(), SpOp::RES, b);  // r = b - A x  (pipelinedCGs.cu:101-103)
+    R.mpk(r.get(), sP2.get());
+    copy_d2d(c, UR2m.get(), sP2.get(), sizeof(double) * bs);
+    copy_d2d(c, UQ2m.get(), sP2.get() + (R.precon ? bs : (size_t)n), sizeof(double) * bs);
+    SstepScalars sc(s);
+    ColumnResult res;
+    if (p.stop_criterion == 1) res.delta0 = R.norm(r.get());
+    std::vector<double> vm;
+    const double* al = R.dab.get();
+    const double* be = R.dab.get() + s;
+    int iter, excnlim = -1;
+    for (iter = 0; iter < p.itnlim; iter++) {
+        bool even = iter % 2 == 0;
+        double* URn = even ? UR2m.get() : UR1m.get();  // the set this iteration builds
+        double* UQn = even ? UQ2m.get() : UQ1m.get();
+        double* URo = even ? UR1m.get() : UR2m.get();  // the previous iteration's
+        double* UQo = even ? UQ1m.get() : UQ2m.get();
+        double* URs = even ? UR1.get() : UR2.get();    // its block 0, saved
+        double* UQs = even ? UQ1.get() : UQ2.get();
+        double* URsn = even ? UR2.get() : UR1.get();
+        double* UQsn = even ? UQ2.get() : UQ1.get();
+        if (iter > 0) {  // pipelinedCGs.cu:150-160 / 218-228
+            for (int j = 1; j < s + 1; j++) {
+                block_dmm(c, n, s, URn + j * bs, be, URo + j * bs);
+                block_dmm(c, n, s, UQn + j * bs, be, UQo + j * bs);
+            }
+            for (int j = 0; j < s; j++) {
+                block_dmv2v(c, n, s, URo + (j + 1) * bs, al, URs + (size_t)j * n, -1.0, URn + (size_t)j * n);
+                block_dmv2v(c, n, s, UQo + (j + 1) * bs, al, UQs + (size_t)j * n, -1.0, UQn + (size_t)j * n);
+            }
+        }
+        pipe_moments(R, URn, UQn, r.get(), vm);
+        copy_d2d(c, ams.get(), UQn + (size_t)(s - 1) * n, sizeof(double) * n);
+        R.mpk(ams.get(), sT.get());
+        int info = sc.step(vm, iter);
+        if (info != 0) {
+            res.retv = info;
+            break;
+        }
+        R.upload(sc);
+        size_t qo = R.precon ? bs : (size_t)n;
+        for (int j = 1; j < s + 1; j++) {  // block j: columns j..s-1 of block 0, then the first j of the new basis
+            copy_d2d(c, URn + j * bs, URn + (size_t)j * n, sizeof(double) * (size_t)(s - j) * n);
+            copy_d2d(c, URn + j * bs + (size_t)(s - j) * n, sT.get(), sizeof(double) * (size_t)j * n);
+            copy_d2d(c, UQn + j * bs, UQn + (size_t)j * n, sizeof(double) * (size_t)(s - j) * n);
+            copy_d2d(c, UQn + j * bs + (size_t)(s - j) * n, sT.get() + qo, sizeof(double) * (size_t)j * n);
+        }
+        copy_d2d(c, URsn, URn, sizeof(double) * bs);
+        copy_d2d(c, UQsn, UQn, sizeof(double) * bs);
+        block_dmmv(c, n, s, URo, be, URn, al, x, 1.0);
+        block_dmmv(c, n, s, UQo, be, UQn, al, r.get(), -1.0);
+        res.l2 = R.norm(r.get());
+        if (res.l2 < p.rtol * res.delta0) {
+            excnlim = 0;
+            res.retv = 0;
+            break;
+        }
+    }
+    res.niter = iter + 1 + excnlim;
+    return res;
+}
+
 }  // namespace
 
 SolveResult cgs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k) {
     BCMGX_CHECK(k >= 1 && k <= 64, BCMGX_ERR_ARG, "nrhs must be in [1, 64]");
-    BCMGX_CHECK(p.solver == BCMGX_SOLVER_CGS, BCMGX_ERR_STATE, "cgs_solve called for another solver");
+    BCMGX_CHECK(p.solver == BCMGX_SOLVER_CGS || p.solver == BCMGX_SOLVER_CGS_CUBLAS || p.solver == BCMGX_SOLVER_PIPELINED_CGS,
+                BCMGX_ERR_STATE, "cgs_solve called for another solver");
     BCMGX_CHECK(p.sstep >= 1 && p.sstep <= 16, BCMGX_ERR_ARG, "sstep must be in [1, 16]");
     double t0 = now_s();
     int n = A.n;
@@ -235,7 +328,8 @@ SolveResult cgs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
             strided_copy(c, bc.get(), 1, b + j, k, n);
             strided_copy(c, xc.get(), 1, x + j, k, n);
         }
-        ColumnResult cr = cgs_column(R, p, bc.get(), xc.get());
+        ColumnResult cr = p.solver == BCMGX_SOLVER_PIPELINED_CGS ? pipecgs_column(R, p, bc.get(), xc.get())
+                                                                 : cgs_column(R, p, bc.get(), xc.get());
         if (k == 1) copy_d2d(c, x, xc.get(), sizeof(double) * n);
         else strided_copy(c, x + j, k, xc.get(), 1, n);
         double rel = cr.delta0 > 0 ? cr.l2 / cr.delta0 : 0.0;

@@ -1332,6 +1332,123 @@ int or_cgs(const ocsr* Ao, const oprec* pr, const bcmgx_params* pp, const double
     return retval;
 }
 
+// cdmm (op/basic.cu:163-183): C = C + (A B), A n x s, B s x s
+static void sstep_dmm(const double* Am, const std::vector<double>& B, double* Cm, int n, int s)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < s; j++) {
+            double ro = 0.0;
+            for (int k = 0; k < s; k++) ro += Am[(size_t)k * n + i] * B[k + s * j];
+            Cm[(size_t)j * n + i] += ro;
+        }
+}
+
+// cdmv2v (op/basic.cu:23-34): D = C + f (A b)
+static void sstep_dmv2v(const double* Am, const std::vector<double>& bv, const double* Cv, double f, double* D, int n, int s)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double ro = 0.0;
+        for (int k = 0; k < s; k++) ro += Am[(size_t)k * n + i] * bv[k];
+        D[i] = Cv[i] + f * ro;
+    }
+}
+
+// pipecomputevm (pipelinedCGs.cu:7-27)
+static void pipe_vm(const double* UR, const double* UQ, int s, const double* r, int n, bool prec, std::vector<double>& vm)
+{
+    vm.assign(2 * s, 0.0);
+    const double* q = UQ + (size_t)(s - 1) * n;
+    for (int j = 0; j < s; j++) vm[j] = dot(UR + (size_t)j * n, r, n);
+    if (prec) {
+        for (int j = 0; j < s; j++) vm[s + j] = dot(UR + (size_t)j * n, q, n);
+    } else {
+        vm[s] = dot(q, r, n);
+        for (int j = 0; j < s - 1; j++) vm[s + 1 + j] = dot(UQ + (size_t)j * n, q, n);
+    }
+}
+
+// pipelinedcgsstep (solver/pipelined_cgs/pipelinedCGs.cu:56-300).  URm/UQm
+// hold s+1 blocks of n x s (block j at j*s*n); the "1" and "2" sets swap
+// roles between even and odd iterations.
+int or_pipecgs(const ocsr* Ao, const oprec* pr, const bcmgx_params* pp, const double* b, double* x,
+               int* niter_out, double* exit_res, double* del0_out)
+{
+    const Csr& A = Ao->A;
+    const bcmgx_params& p = *pp;
+    int n = A.n, s = std::max(1, p.sstep);
+    bool prec = p.preconditioner != BCMGX_PREC_NONE;
+    size_t bs = (size_t)n * s;  // one n x s block
+    std::vector<double> r(b, b + n), w(n), sP2((size_t)n * 2 * s, 0.0), sT((size_t)n * 2 * s, 0.0);
+    std::vector<double> UR1m(bs * (s + 1), 0.0), UR2m(bs * (s + 1), 0.0), UQ1m(bs * (s + 1), 0.0), UQ2m(bs * (s + 1), 0.0);
+    std::vector<double> UR1(bs, 0.0), UR2(bs, 0.0), UQ1(bs, 0.0), UQ2(bs, 0.0);
+    spmv(A, x, w.data(), 1.0, 0.0, 1);
+    for (int i = 0; i < n; i++) r[i] = -1.0 * w[i] + 1.0 * r[i];
+    std::vector<double> vm, alpha(s, 0.0), beta(s * s, 0.0), W(s * s, 0.0);
+    sstep_mpk(A, pr, prec, r.data(), s, sP2.data());
+    std::copy(sP2.begin(), sP2.begin() + bs, UR2m.begin());
+    std::copy(sP2.begin() + (prec ? bs : n), sP2.begin() + (prec ? bs : n) + bs, UQ2m.begin());
+    double delta0 = 1.0, l2 = 0.0;
+    if (p.stop_criterion == 1) delta0 = std::sqrt(dot(r.data(), r.data(), n));
+    int iter, excnlim = -1, retval = -1;
+    for (iter = 0; iter < p.itnlim; iter++) {
+        bool even = iter % 2 == 0;
+        // even: the new set is 2, the previous 1 (odd: the reverse)
+        std::vector<double>& URn = even ? UR2m : UR1m;
+        std::vector<double>& UQn = even ? UQ2m : UQ1m;
+        std::vector<double>& URo = even ? UR1m : UR2m;
+        std::vector<double>& UQo = even ? UQ1m : UQ2m;
+        std::vector<double>& URs = even ? UR1 : UR2;  // block 0 saved by the previous iteration
+        std::vector<double>& UQs = even ? UQ1 : UQ2;
+        std::vector<double>& URsn = even ? UR2 : UR1;
+        std::vector<double>& UQsn = even ? UQ2 : UQ1;
+        if (iter > 0) {
+            // pipelinedCGs.cu:150-160 (even), 218-228 (odd)
+            for (int j = 1; j < s + 1; j++) {
+                sstep_dmm(URn.data() + j * bs, beta, URo.data() + j * bs, n, s);
+                sstep_dmm(UQn.data() + j * bs, beta, UQo.data() + j * bs, n, s);
+            }
+            for (int j = 0; j < s; j++) {
+                sstep_dmv2v(URo.data() + (j + 1) * bs, alpha, URs.data() + (size_t)j * n, -1.0, URn.data() + (size_t)j * n, n, s);
+                sstep_dmv2v(UQo.data() + (j + 1) * bs, alpha, UQs.data() + (size_t)j * n, -1.0, UQn.data() + (size_t)j * n, n, s);
+            }
+        }
+        pipe_vm(URn.data(), UQn.data(), s, r.data(), n, prec, vm);
+        std::vector<double> ams(UQn.begin() + (size_t)(s - 1) * n, UQn.begin() + (size_t)s * n);
+        sstep_mpk(A, pr, prec, ams.data(), s, sT.data());
+        int info = sstep_scalar(vm, W, alpha, beta, s, iter);
+        if (info != 0) {
+            retval = info;
+            break;
+        }
+        for (int j = 1; j < s + 1; j++) {  // block j: columns j..s-1 of block 0, then the first j of the new basis
+            double* Rj = URn.data() + j * bs;
+            double* Qj = UQn.data() + j * bs;
+            std::copy(URn.begin() + (size_t)j * n, URn.begin() + (size_t)s * n, Rj);
+            std::copy(sT.begin(), sT.begin() + (size_t)j * n, Rj + (size_t)(s - j) * n);
+            std::copy(UQn.begin() + (size_t)j * n, UQn.begin() + (size_t)s * n, Qj);
+            size_t qo = prec ? bs : (size_t)n;
+            std::copy(sT.begin() + qo, sT.begin() + qo + (size_t)j * n, Qj + (size_t)(s - j) * n);
+        }
+        std::copy(URn.begin(), URn.begin() + bs, URsn.begin());
+        std::copy(UQn.begin(), UQn.begin() + bs, UQsn.begin());
+        sstep_dmmv(URo.data(), beta, URn.data(), alpha, x, 1.0, n, s);
+        sstep_dmmv(UQo.data(), beta, UQn.data(), alpha, r.data(), -1.0, n, s);
+        l2 = std::sqrt(dot(r.data(), r.data(), n));
+        if (p.dispnorm > 1) printf("%d) r norm: %.10lf\n", iter, l2);
+        if (l2 < p.rtol * delta0) {
+            excnlim = 0;
+            retval = 0;
+            break;
+        }
+    }
+    *niter_out = iter + 1 + excnlim;
+    *exit_res = l2;
+    *del0_out = delta0;
+    return retval;
+}
+
 int or_num_threads(void)
 {
 #ifdef _OPENMP

@@ -71,6 +71,9 @@ int or_cghs(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double*
 // cgsstep (CGs.cu:43-236), s = the sstep setting; same outputs
 int or_cgs(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
            int* niter, double* exit_res, double* del0);
+// pipelinedcgsstep (pipelinedCGs.cu:56-300); same outputs
+int or_pipecgs(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,
+               int* niter, double* exit_res, double* del0);
 
 int or_num_threads(void);
 

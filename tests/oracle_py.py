@@ -67,6 +67,8 @@ def lib():
         L.or_cghs.argtypes = L.or_fcg.argtypes
         L.or_cgs.restype = C.c_int
         L.or_cgs.argtypes = L.or_fcg.argtypes
+        L.or_pipecgs.restype = C.c_int
+        L.or_pipecgs.argtypes = L.or_fcg.argtypes
         L.or_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -222,6 +224,15 @@ class Prec:
         u = np.zeros(n) if u is None else np.ascontiguousarray(u, np.float64).copy()
         lib().or_prec_apply(self.h, np.ascontiguousarray(r, np.float64), u)
         return u
+
+
+def pipecgs(A, prec, params, b, x0=None):
+    n, _, _ = A.dims()
+    x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    it, er, d0 = C.c_int(), C.c_double(), C.c_double()
+    retv = lib().or_pipecgs(A.h, prec.h, C.byref(params), np.ascontiguousarray(b, np.float64), x, C.byref(it), C.byref(er),
+                            C.byref(d0))
+    return x, dict(retv=retv, niter=it.value, exit_res=er.value, del0=d0.value)
 
 
 def cgs(A, prec, params, b, x0=None):

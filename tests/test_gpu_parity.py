@@ -508,13 +508,39 @@ def test_sstep_cg_options_columns_and_breakdown(ctx):
     assert out["retv"] > 0
 
 
-def test_unsupported_solver_is_rejected(ctx):
+@pytest.mark.parametrize("solver", ["PIPELINED_CGS", "CGS_CUBLAS"])
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI", "BCMG", "AFSAI"])
+@pytest.mark.parametrize("s", [1, 3])
+def test_pipelined_and_cublas_sstep_match_oracle(ctx, solver, prec, s):
+    """pipelinedCGs.cu against the pipelined oracle; CGscublas.cu (the s-step method on BLAS
+    calls) against the s-step oracle."""
+    oa = CASES["lap27_10"]() if prec != "NONE" else CASES["lap7_12"]()
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    p = params(prec, solver=solver, sstep=s, rtol=1e-9, itnlim=3000)
+    b = np.random.default_rng(19).normal(size=n)
+    ref = O.pipecgs if solver == "PIPELINED_CGS" else O.cgs
+    q = oparams(p)
+    if solver == "CGS_CUBLAS":
+        q.solver = abi.SOLVER["CGS"]
+    xo, io = ref(oa, O.Prec(oa, q), q, b)
+    P = bx.prec_setup(ctx, A, p)
+    x, out = bx.solve(ctx, A, b, None, p, P)
+    P.finalize()
+    assert out["retv"] == io["retv"] == 0
+    assert abs(out["niter"] - io["niter"]) <= 1
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_unknown_solver_is_rejected(ctx):
+    """An unknown solver name parses to INVALID and solve() refuses it (solve.cu:42-44)."""
     oa = CASES["lap7_12"]()
     A = dev(ctx, oa)
     p = params("BCMG")
     P = bx.prec_setup(ctx, A, p)
-    p.set("solver", "PIPELINED_CGS")
+    p.set("solver", "GMRES")
+    assert p.solver == abi.SOLVER["INVALID"]
     with pytest.raises(bx.BcmgxError) as e:
         bx.solve(ctx, A, np.ones(A.n), None, p, P)
-    assert e.value.status == -6 and "FCG" in str(e.value)
+    assert e.value.status == -6 and "SolverType" in str(e.value)
     P.finalize()

@@ -447,6 +447,42 @@ def test_sstep_residual_options_and_breakdown():
     assert inn["retv"] > 0
 
 
+# ------------------------------------------- pipelined s-step CG (pipelinedCGs.cu)
+def pipe_params(prec, s, **kw):
+    p = cgs_params(prec, s, **kw)
+    p.solver = abi.SOLVER["PIPELINED_CGS"]
+    return p
+
+
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI", "BCMG", "AFSAI"])
+@pytest.mark.parametrize("s", [1, 2, 3])
+def test_pipelined_sstep_equals_sstep(prec, s):
+    """The pipelined form re-associates the s-step recurrences (the basis of the next block is
+    recombined from the current one): same outer iterations and solution as CGs, and PCG's for s = 1."""
+    A = O.laplacian3d(7, 11, 10, 12)
+    b = np.random.default_rng(6).normal(size=A.dims()[0])
+    ps = cgs_params(prec, s, rtol=1e-9, itnlim=3000)
+    pp = pipe_params(prec, s, rtol=1e-9, itnlim=3000)
+    xs, is_ = O.cgs(A, O.Prec(A, ps), ps, b)
+    xp, ip = O.pipecgs(A, O.Prec(A, pp), pp, b)
+    assert ip["retv"] == 0 and abs(ip["niter"] - is_["niter"]) <= 1
+    assert np.linalg.norm(xp - xs) / np.linalg.norm(xs) < 1e-7
+    if s == 1:
+        pc = cghs_params(prec, rtol=1e-9, itnlim=3000)
+        _, ic = O.cghs(A, O.Prec(A, pc), pc, b)
+        assert abs(ip["niter"] - ic["niter"]) <= 1
+
+
+@pytest.mark.parametrize("pts", [7, 27])
+def test_pipelined_sstep_matches_reference_fixture(golden, pts):
+    A = O.laplacian3d(pts, 10, 10, 10)
+    p = pipe_params("BCMG", 2)
+    x, info = O.pipecgs(A, O.Prec(A, p), p, np.ones(1000))
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_1x1x1_sol.txt"))
+    assert info["retv"] == 0
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
+
+
 # -------------------------------------------------------------------- AFSAI
 def test_afsai_factor_invariants():
     A = O.laplacian3d(7, 8, 7, 6)
