@@ -506,7 +506,7 @@ def prec_finalize(prec):
 
 # ------------------------------------------------------------------- solve ---
 def solve(ctx, A, rhs, x0, params, prec=None):
-    """solve (solve.h:33) -> FCG (FCG.cu:27-175).
+    """solve (solve.h:33): FCG, CGHS, CGS, CGS_CUBLAS or PIPELINED_CGS per ``params.solver``.
 
     ``rhs`` and ``x0`` are (n,) or (n, k) arrays; returns (x, SolverOut dict),
     the SolverOut with the reference's counting (niter = iterations + 1).

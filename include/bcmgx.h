@@ -189,18 +189,27 @@ int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* p
 int bcmgx_prec_afsai_factor(bcmgx_prec prec, int* n, int* nnz, int* row, int* col, double* val);
 
 /* ---- solver ---------------------------------------------------------------
- * solve (src/solver/solve.h:33) -> solve_fcg (src/solver/fcg/FCG.h:37).
+ * solve (src/solver/solve.h:33, dispatch solve.cu:19-40) on p->solver:
+ *   FCG           solve_fcg (src/solver/fcg/FCG.cu:27-215)
+ *   CGHS          solve_cg_hs (src/solver/cghs/CG_HS.cu:24-192)
+ *   CGS           solve_cgs, s-step CG with s = p->sstep (src/solver/cgs/CGs.cu:43-236)
+ *   CGS_CUBLAS    solve_cgs_cublas, the same s-step method (src/solver/cgs_cublas/CGscublas.cu)
+ *   PIPELINED_CGS solve_pipelined_cgs (src/solver/pipelined_cgs/pipelinedCGs.cu:56-329)
  * x is in/out (initial guess, then solution), as the reference's x0.
  * nrhs >= 1 columns (column-major, leading dimension ld) share one
- * preconditioner; each column keeps its own FCG scalars and stops on its own
- * criterion ||r_k|| <= rtol ||r_0|| (FCG.cu:93,156).
+ * preconditioner.  FCG and CGHS solve them together, each column with its own
+ * scalars and stopping test; the s-step forms solve them one after another.
+ * Stopping: FCG ||r_k|| <= rtol ||r_0|| (FCG.cu:93,156); the others
+ * ||r_k|| < rtol delta0 with delta0 = ||r_0|| for stop_criterion 1, else 1
+ * (CG_HS.cu:74-79,141).
  */
 typedef struct bcmgx_solver_out {
-    int retv;               /* 0 converged, -1 itnlim reached (FCG.cu:160-162); worst column */
-    int niter;              /* reference counting (FCG.cu:164); max over columns */
+    int retv;               /* 0 converged, -1 itnlim reached (FCG.cu:160-162), > 0 the s-step
+                               s x s system was not positive definite (scalarWorkMO); worst column */
+    int niter;              /* the reference's counting (FCG.cu:164, CG_HS.cu:168); max over columns */
     double exit_res;        /* ||r|| at exit (FCG.cu:165); max over columns */
-    double rel_res;         /* exit_res / ||r_0||; max over columns */
-    double del0;            /* ||r_0|| of column 0 */
+    double rel_res;         /* exit_res / delta0; max over columns */
+    double del0;            /* delta0 of column 0 (||r_0||, or 1 for an absolute test) */
     double solve_seconds;   /* device time of the solve */
     int full_n;
     int local_n;
