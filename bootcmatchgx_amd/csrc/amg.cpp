@@ -29,10 +29,10 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     auto it = c->tuned_lanes.find(sig);
     if (it != c->tuned_lanes.end()) return it->second;
     int base = default_lanes(A.nnz, A.n);
-    // vector kernels with L lanes per row, and CSR-stream kernels (negative: lanes per row reduction)
     // vector kernels (L lanes per row), CSR-stream kernels (-TPR lanes per row
-    // reduction) and their paired-load form (-200 - TPR)
-    int cands[8] = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8, -204, -208};
+    // reduction), their paired-load form (-200 - TPR), and both with rows dealt
+    // to the XCDs in super-blocks of 32 blocks (-6000 - ...)
+    int cands[10] = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8, -204, -208, -6004, -6204};
     hipEvent_t e0, e1;
     BCMGX_HIP(hipEventCreate(&e0));
     BCMGX_HIP(hipEventCreate(&e1));

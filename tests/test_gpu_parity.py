@@ -399,6 +399,31 @@ def test_graph_replay_equals_eager(ctx):
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][0], res[2][0])
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_cghs_graph_replay_and_solver_switch(ctx, precision):
+    """CG from its replayed iteration graph gives the eager bits; FCG and CG alternate on one
+    preconditioner and workspace without disturbing each other."""
+    oa = O.laplacian3d(27, 16, 16, 16)
+    b = O.spmv(oa, np.random.default_rng(21).uniform(-1, 1, 4096))
+    A = dev(ctx, oa)
+    res = []
+    for g in (1, 0):
+        p = params("BCMG", solver="CGHS", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, precision=precision,
+                   use_graph=g)
+        P = bx.prec_setup(ctx, A, p)
+        x1, o1 = bx.solve(ctx, A, b, None, p, P)
+        x2, o2 = bx.solve(ctx, A, b, None, p, P)
+        assert o1["retv"] == 0 and o1["niter"] == o2["niter"] and np.array_equal(x1, x2)
+        xf, of = bx.solve(ctx, A, b, None, p.copy().set("solver", "FCG"), P)
+        x3, o3 = bx.solve(ctx, A, b, None, p, P)
+        assert np.array_equal(x1, x3)
+        for x in (x1, xf):
+            assert np.linalg.norm(b - O.spmv(oa, x)) <= 1e-8 * np.linalg.norm(b) * 1.0000001
+        res.append((x1, o1["niter"]))
+        P.finalize()
+    assert res[0][1] == res[1][1] and np.array_equal(res[0][0], res[1][0])
+
+
 def test_itnlim_and_info_file(ctx, tmp_path):
     oa = O.laplacian3d(7, 12, 12, 12)
     A = dev(ctx, oa)
