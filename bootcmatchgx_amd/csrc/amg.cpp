@@ -590,13 +590,13 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
     a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
     if (s.mv_blocks > 0 && spmm_dots(c, a, s.r.get(), W1, s.fb.partials, s.mv_blocks, s.fb.all_done)) {
-        fcg_scalars3(c, s.fb, s.mv_blocks);
+        fcg_scalars3(c, s.fb, s.mv_blocks, idx);
     } else {
         spmv<double, double>(c, SpOp::AX, a);
-        fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1);
+        fcg_dots3(c, s.fb, s.r.get(), P0, W0, W1, idx);
     }
-    fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get(), s.entry);
-    fcg_finish_iteration(c, s.fb);
+    fcg_update(c, s.fb, P0, P1, W0, W1, s.X.get(), s.r.get(), idx, s.entry);
+    fcg_finish_iteration(c, s.fb, idx);
 }
 
 // the Krylov workspace for k columns of A (kept per k: graphs are bound to it)
@@ -695,6 +695,9 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         if (hflag[0]) break;
         if (launched > p.itnlim + 2) break;  // the device stops every column at itnlim
     }
+    // every column stopped after the first iteration of the last pair: its x step waits in pend
+    // and its direction is still in v (the first iteration's p0)
+    fcg_flush_pending(c, s.fb, s.v.get(), s.X.get());
     copy_d2d(c, x, s.X.get(), sizeof(double) * nk);
     std::vector<FcgState> st(k);
     c->d2h(st.data(), s.st.get(), k);

@@ -107,6 +107,8 @@ __global__ __launch_bounds__(WG) void k_norm_final(const double* __restrict__ pa
             st[c].delta0 = l2;
             st[c].iter = 0;
             st[c].done = 0;
+            st[c].pend = 0.0;
+            st[c].pad1 = 0;
             if (l2 == 0.0) {  // b == A x already: nothing to iterate on
                 st[c].done = 1;
                 if (atomicAdd(ndone, 1) + 1 == k) *all_done = 1;
@@ -180,7 +182,7 @@ void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const d
 }
 
 __global__ __launch_bounds__(WG) void k_scalars3(const double* __restrict__ partials, int nblk, int k, FcgState* st,
-                                                 const int* __restrict__ all_done) {
+                                                 const int* __restrict__ all_done, int phase) {
     if (*all_done) return;
     __shared__ double s[3][WG];
     double tot[3];
@@ -195,24 +197,33 @@ __global__ __launch_bounds__(WG) void k_scalars3(const double* __restrict__ part
         S.theta = S.gamma / S.delta;                        // FCG.cu:132
         S.delta = S.beta - ((S.gamma * S.gamma) / S.delta);  // FCG.cu:135
         S.theta2 = S.alpha / S.delta;                       // FCG.cu:136
+        if (phase == 0) {  // the x step of this iteration waits for the pair's second update
+            S.pend = S.theta2;
+            S.pad1 = 1;
+        }
     }
 }
 
-void fcg_scalars3(Context* c, const FcgBufs& f, int nblk) {
-    k_scalars3<<<f.k, WG, 0, c->stream>>>(f.partials, nblk, f.k, f.st, f.all_done);
+void fcg_scalars3(Context* c, const FcgBufs& f, int nblk, int phase) {
+    k_scalars3<<<f.k, WG, 0, c->stream>>>(f.partials, nblk, f.k, f.st, f.all_done, phase);
     BCMGX_LAUNCHED();
 }
 
-void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1) {
+void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1, int phase) {
     int KP = kpad(f.k);
     k_dots<3><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, p0, w0, p0, w1, p0, f.partials, f.all_done);
     BCMGX_LAUNCHED();
-    k_scalars3<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.all_done);
+    k_scalars3<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.all_done, phase);
     BCMGX_LAUNCHED();
 }
 
 // double_merged_axpy(P1, P0, x, -theta, theta2); double_merged_axpy(W1, W0, r, -theta, -theta2)
-// (double_merged_axpy.cu:28-39), fused, with the partial sums of r.r.
+// (double_merged_axpy.cu:28-39), fused, with the partial sums of r.r.  The x
+// step is taken every second iteration: phase 0 leaves x alone (its theta2
+// waits in pend, its direction stays in the buffer that is p1 next time),
+// phase 1 forms x = theta2' t' + (pend p1 + x) -- the two steps in the
+// reference's order, so x has the same bits -- and saves one x read and write.
+template <int PHASE>
 __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgState* __restrict__ st,
                                                double* __restrict__ p0, const double* __restrict__ p1,
                                                double* __restrict__ w0, const double* __restrict__ w1,
@@ -228,12 +239,19 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
     if (c < k) {
         const FcgState S = st[c];
         double mth = -S.theta, th2 = S.theta2, mth2 = -S.theta2;
+        bool waiting = PHASE == 1 && S.pad1;  // the pair's first step, still to be added
+        double pend = S.pend;
         for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
             long long o = i * k + c;
             double rr;
             if (!S.done) {
-                double t = mth * p1[o] + p0[o];
-                x[o] = th2 * t + x[o];
+                double p1o = p1[o];
+                double t = mth * p1o + p0[o];
+                if constexpr (PHASE == 1) {
+                    double xo = x[o];
+                    double xw = pend * p1o + xo;
+                    x[o] = th2 * t + (waiting ? xw : xo);
+                }
                 p0[o] = t;
                 double u = mth * w1[o] + w0[o];
                 rr = mth2 * u + r[o];
@@ -241,6 +259,7 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
                 w0[o] = u;
             } else {
                 rr = r[o];
+                if (waiting) x[o] = pend * p1[o] + x[o];
             }
             acc += rr * rr;
             if (e.F) {
@@ -260,15 +279,16 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
 }
 
 void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r,
-                const FcgEntry& e) {
+                int phase, const FcgEntry& e) {
     int KP = kpad(f.k);
-    k_update<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p0, p1, w0, w1, x, r, f.partials, f.all_done, e);
+    if (phase == 0) k_update<0><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p0, p1, w0, w1, x, r, f.partials, f.all_done, e);
+    else k_update<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, p0, p1, w0, w1, x, r, f.partials, f.all_done, e);
     BCMGX_LAUNCHED();
 }
 
 // l2 = ||r||; iter++; stop when !(l2 > rtol*delta0 && iter < itnlim)  (FCG.cu:146-156)
 __global__ __launch_bounds__(WG) void k_finish(const double* __restrict__ partials, int nblk, int k, FcgState* st,
-                                               double rtol, int itnlim, int* all_done, int* ndone) {
+                                               double rtol, int itnlim, int* all_done, int* ndone, int phase) {
     if (*all_done) return;
     __shared__ double s[1][WG];
     double tot[1];
@@ -276,6 +296,7 @@ __global__ __launch_bounds__(WG) void k_finish(const double* __restrict__ partia
     sum_partials<1>(partials, nblk, k, c, tot, s);
     if (threadIdx.x == 0) {
         FcgState& S = st[c];
+        if (phase == 1) S.pad1 = 0;  // this pair's update added the waiting step
         if (S.done) return;
         S.l2 = sqrt(tot[0]);
         S.iter += 1;
@@ -286,8 +307,34 @@ __global__ __launch_bounds__(WG) void k_finish(const double* __restrict__ partia
     }
 }
 
-void fcg_finish_iteration(Context* c, const FcgBufs& f) {
-    k_finish<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.rtol, f.itnlim, f.all_done, f.all_done + 1);
+void fcg_finish_iteration(Context* c, const FcgBufs& f, int phase) {
+    k_finish<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, f.rtol, f.itnlim, f.all_done, f.all_done + 1, phase);
+    BCMGX_LAUNCHED();
+}
+
+__global__ __launch_bounds__(WG) void k_flush_pending(int n, int k, int KP, FcgState* st, const double* __restrict__ t,
+                                                      double* __restrict__ x) {
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    if (c >= k || !st[c].pad1) return;
+    double pend = st[c].pend;
+    for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+        long long o = i * k + c;
+        x[o] = pend * t[o] + x[o];
+    }
+}
+
+__global__ void k_clear_pending(int k, FcgState* st) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < k) st[c].pad1 = 0;
+}
+
+void fcg_flush_pending(Context* c, const FcgBufs& f, const double* t, double* x) {
+    int KP = kpad(f.k);
+    k_flush_pending<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, t, x);
+    BCMGX_LAUNCHED();
+    k_clear_pending<<<1, 64, 0, c->stream>>>(f.k, f.st);
     BCMGX_LAUNCHED();
 }
 

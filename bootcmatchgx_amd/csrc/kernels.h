@@ -79,7 +79,8 @@ void convert_stop(Context* c, const A* s, B* d, long long n, const int* stop);
 // ---- FCG kernels (fcg.hip) ----------------------------------------------------
 struct FcgState {
     double delta0, delta, l2, alpha, beta, gamma, theta, theta2;
-    int iter, done, pad0, pad1;
+    double pend;  // FCG: theta2 of the first iteration of a pair, applied to x by the second
+    int iter, done, pad0, pad1;  // FCG: pad1 = 1 while pend waits; CG: pad0 = converged
 };
 struct FcgBufs {
     int n = 0, k = 1;
@@ -107,14 +108,19 @@ void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* 
 void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q,
                      const FcgEntry& e = FcgEntry());
 // (alpha, beta, gamma) = (r.p0, w0.p0, w1.p0), then theta, delta, theta2 (FCG.cu:113-136)
-void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1);
-// the scalar step of fcg_dots3 alone, over `nblk` partials already written
-void fcg_scalars3(Context* c, const FcgBufs& f, int nblk);
-// double_merged_axpy x2 (FCG.cu:138-144) fused with the ||r||^2 partials
+void fcg_dots3(Context* c, const FcgBufs& f, const double* r, const double* p0, const double* w0, const double* w1, int phase);
+// the scalar step of fcg_dots3 alone, over `nblk` partials already written.
+// phase: 0 for the first iteration of a pair (its x step waits), 1 for the second
+void fcg_scalars3(Context* c, const FcgBufs& f, int nblk, int phase);
+// double_merged_axpy x2 (FCG.cu:138-144) fused with the ||r||^2 partials.  Phase 0
+// leaves x alone (its step waits in pend); phase 1 adds both steps to x in order.
 void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, double* w0, const double* w1, double* x, double* r,
-                const FcgEntry& e = FcgEntry());
+                int phase, const FcgEntry& e = FcgEntry());
 // ||r|| -> convergence test and iteration count (FCG.cu:146-156)
-void fcg_finish_iteration(Context* c, const FcgBufs& f);
+void fcg_finish_iteration(Context* c, const FcgBufs& f, int phase);
+// a step still waiting after the last pair (every column stopped after its
+// first iteration): x += pend t, t the first iteration's direction
+void fcg_flush_pending(Context* c, const FcgBufs& f, const double* t, double* x);
 
 // ---- CG (Hestenes-Stiefel) kernels (fcg.hip), CG_HS.cu:24-172 ----------------
 // ||r|| -> delta0 (||r|| for stop_criterion 1, else 1), iteration count 0

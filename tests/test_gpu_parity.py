@@ -424,6 +424,27 @@ def test_cghs_graph_replay_and_solver_switch(ctx, precision):
     assert res[0][1] == res[1][1] and np.array_equal(res[0][0], res[1][0])
 
 
+@pytest.mark.parametrize("itnlim", [2, 3, 4, 5])
+@pytest.mark.parametrize("k", [1, 4])
+def test_fcg_stop_after_either_iteration_of_a_pair(ctx, itnlim, k):
+    """x takes its FCG step every second iteration; stopping after the first iteration of a pair
+    (odd itnlim) must still add the waiting step: x equals the oracle's sequential updates."""
+    oa = O.laplacian3d(7, 14, 12, 10)
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    p = params("BCMG", rtol=1e-30, itnlim=itnlim)
+    P = bx.prec_setup(ctx, A, p)
+    B = np.random.default_rng(23).normal(size=(n, k))
+    X, out = bx.solve(ctx, A, B, None, p, P)
+    X = X.reshape(n, k)
+    OP = O.Prec(oa, oparams(p))
+    for j in range(k):
+        xo, io = O.fcg(oa, OP, oparams(p), B[:, j])
+        assert out["retv"] == io["retv"] == -1
+        assert np.linalg.norm(X[:, j] - xo) / np.linalg.norm(xo) < 1e-12
+    P.finalize()
+
+
 def test_itnlim_and_info_file(ctx, tmp_path):
     oa = O.laplacian3d(7, 12, 12, 12)
     A = dev(ctx, oa)
