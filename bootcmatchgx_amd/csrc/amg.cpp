@@ -143,7 +143,8 @@ static int reference_max_coarse_size(long long n) { return (int)(40 * std::pow((
 static void check_diagonals(Context* c, const DCsr& A) {
     DBuf<int> dind(c, std::max(A.n, 1));
     diag_index(c, A.n, A.row.get(), A.col.get(), dind.get());
-    std::vector<int> h(A.n);
+    if (count_missing_diagonals(c, A.n, dind.get()) == 0) return;
+    std::vector<int> h(A.n);  // name the first offending row
     c->d2h(h.data(), dind.get(), A.n);
     for (int i = 0; i < A.n; i++)
         BCMGX_CHECK(h[i] >= 0, BCMGX_ERR_ARG, "row " + std::to_string(i) + " has no diagonal entry (required by the matching)");

@@ -149,6 +149,8 @@ void strided_copy(Context* c, double* dst, int ds, const double* src, int ss, in
 // ---- setup kernels (setup.hip) -------------------------------------------------
 // position of the first diagonal entry of each row (-1 if none)
 void diag_index(Context* c, int n, const int* row, const int* col, int* dind);
+// rows with no diagonal entry (dind < 0), counted on the device
+int count_missing_diagonals(Context* c, int n, const int* dind);
 // M_i = sum_j |a_ij| in column order (CSRm::absoluteRowSum, CSR.cu:1362-1397)
 void abs_row_sum(Context* c, int n, const int* row, const double* val, double* M);
 // l1 diagonal of the one-level preconditioner: a_ii + sum_{j!=i} |a_ij| (l1jacobi.cu:484-493)

@@ -376,6 +376,15 @@ static int lanes_for(long long nnz, int n) {
     return 32;
 }
 
+int count_missing_diagonals(Context* c, int n, const int* dind) {
+    if (n <= 0) return 0;
+    DBuf<int> miss(c, 1);
+    fill<int>(c, miss.get(), 1, 0);
+    k_count_missing_diag<<<grid_for(n, WG), WG, 0, c->stream>>>(n, dind, miss.get());
+    BCMGX_LAUNCHED();
+    return c->read1(miss.get());
+}
+
 void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     int n = A.n;
     if (n <= 0) return;
@@ -383,13 +392,8 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     size_t mm = (size_t)std::max<long long>(m, 1);
     DBuf<int> dind(c, n);
     diag_index(c, n, A.row.get(), A.col.get(), dind.get());
-    {  // makeAH requires a diagonal entry in every row (matching.cu:155)
-        DBuf<int> miss(c, 1);
-        fill<int>(c, miss.get(), 1, 0);
-        k_count_missing_diag<<<grid_for(n, WG), WG, 0, c->stream>>>(n, dind.get(), miss.get());
-        BCMGX_LAUNCHED();
-        BCMGX_CHECK(c->read1(miss.get()) == 0, BCMGX_ERR_ARG, "matching: every row needs a diagonal entry");
-    }
+    // makeAH requires a diagonal entry in every row (matching.cu:155)
+    BCMGX_CHECK(count_missing_diagonals(c, n, dind.get()) == 0, BCMGX_ERR_ARG, "matching: every row needs a diagonal entry");
     DBuf<double> C(c, n);
     k_make_C<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.val.get(), dind.get(), w, C.get());
     BCMGX_LAUNCHED();

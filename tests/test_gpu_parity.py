@@ -445,6 +445,19 @@ def test_fcg_stop_after_either_iteration_of_a_pair(ctx, itnlim, k):
     P.finalize()
 
 
+def test_missing_diagonal_is_reported(ctx):
+    """The matching needs a diagonal entry in every row (matching.cu:155): setup names the row."""
+    S = sp.diags([np.full(40, 4.0), np.full(39, -1.0), np.full(39, -1.0)], [0, -1, 1], format="lil")
+    S[7, 7] = 0.0
+    S = S.tocsr()
+    S.eliminate_zeros()
+    S.sort_indices()
+    A = dev(ctx, O.Csr.from_scipy(S))
+    with pytest.raises(bx.BcmgxError) as e:
+        bx.prec_setup(ctx, A, params("BCMG"))
+    assert "row 7 has no diagonal" in str(e.value)
+
+
 def test_itnlim_and_info_file(ctx, tmp_path):
     oa = O.laplacian3d(7, 12, 12, 12)
     A = dev(ctx, oa)
