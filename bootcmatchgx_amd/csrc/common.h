@@ -123,6 +123,17 @@ struct Context {
     // copy n elements device->host through pinned memory and wait
     template <typename T>
     void d2h(T* dst, const T* src, size_t n);
+    // two small device->host reads with one synchronisation
+    template <typename A, typename B>
+    void d2h2(A* a, const A* da, size_t na, B* b, const B* db, size_t nb) {
+        size_t oa = na * sizeof(A), ob = (oa + 15) / 16 * 16;
+        char* h = (char*)host_scratch(ob + nb * sizeof(B));
+        BCMGX_HIP(hipMemcpyAsync(h, da, oa, hipMemcpyDeviceToHost, stream));
+        BCMGX_HIP(hipMemcpyAsync(h + ob, db, nb * sizeof(B), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::memcpy(a, h, oa);
+        std::memcpy(b, h + ob, nb * sizeof(B));
+    }
     template <typename T>
     T read1(const T* src) {
         T v;

@@ -11,6 +11,9 @@ namespace bcmgx {
 int scan_exclusive(Context* c, const int* in, int* out, long long n);
 long long scan_exclusive(Context* c, const long long* in, long long* out, long long n);
 long long scan_exclusive_i32_to_i64(Context* c, const int* in, long long* out, long long n);
+// the same, the total left in out[n] on the device (no host synchronisation)
+void scan_exclusive_async(Context* c, const int* in, int* out, long long n);
+void scan_exclusive_async(Context* c, const long long* in, long long* out, long long n);
 template <typename T>
 void fill(Context* c, T* p, long long n, T v);
 template <typename A, typename B>

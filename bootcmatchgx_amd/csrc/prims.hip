@@ -182,12 +182,11 @@ __global__ __launch_bounds__(WG) void k_scan_final(const TI* __restrict__ in, lo
 }
 
 template <typename TI, typename T>
-static T scan_impl(Context* c, const TI* in, T* out, long long n) {
-    // out[0..n) exclusive prefix sums, out[n] = total; returns the total.
+static T scan_impl(Context* c, const TI* in, T* out, long long n, bool read_total = true) {
+    // out[0..n) exclusive prefix sums, out[n] = total; returns the total
+    // (read back only when asked: the other callers leave it on the device).
     if (n <= 0) {
-        T z = 0;
-        BCMGX_HIP(hipMemcpyAsync(out, &z, sizeof(T), hipMemcpyHostToDevice, c->stream));
-        c->sync();
+        fill<T>(c, out, 1, (T)0);
         return 0;
     }
     long long nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -198,10 +197,14 @@ static T scan_impl(Context* c, const TI* in, T* out, long long n) {
     BCMGX_LAUNCHED();
     k_scan_final<TI, T><<<(unsigned)nb, WG, 0, c->stream>>>(in, n, part.get(), out);
     BCMGX_LAUNCHED();
-    return c->read1(out + n);
+    return read_total ? c->read1(out + n) : (T)0;
 }
 
 int scan_exclusive(Context* c, const int* in, int* out, long long n) { return scan_impl<int, int>(c, in, out, n); }
+void scan_exclusive_async(Context* c, const int* in, int* out, long long n) { scan_impl<int, int>(c, in, out, n, false); }
+void scan_exclusive_async(Context* c, const long long* in, long long* out, long long n) {
+    scan_impl<long long, long long>(c, in, out, n, false);
+}
 long long scan_exclusive(Context* c, const long long* in, long long* out, long long n) {
     return scan_impl<long long, long long>(c, in, out, n);
 }

@@ -495,11 +495,12 @@ __global__ void k_single_flags(int n, const int* __restrict__ M, const int* __re
     inpair[v] = ip;
     single[v] = (!ip && fabs(w[v]) > DBL_EPSILON) ? 1 : 0;
 }
-__global__ void k_assign_singles(int n, int npairs, const int* __restrict__ inpair, const int* __restrict__ sid,
+__global__ void k_assign_singles(int n, const int* __restrict__ npairs_d, const int* __restrict__ inpair, const int* __restrict__ sid,
                                  const double* __restrict__ w, int* __restrict__ agg, double* __restrict__ pval) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     if (inpair[v]) return;
+    int npairs = *npairs_d;
     double wv = w[v];
     if (fabs(wv) > DBL_EPSILON) {
         agg[v] = npairs + sid[v];
@@ -516,14 +517,16 @@ int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg,
     DBuf<int> good(c, n), pid(c, (size_t)n + 1), single(c, n), inpair(c, n), sid(c, (size_t)n + 1);
     k_pair_flags<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, w, good.get(), pval);
     BCMGX_LAUNCHED();
-    int npairs = scan_exclusive(c, good.get(), pid.get(), n);
+    scan_exclusive_async(c, good.get(), pid.get(), n);  // pid[n] = number of pairs, read by the kernels
     k_assign_pairs<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, good.get(), pid.get(), agg);
     BCMGX_LAUNCHED();
     k_single_flags<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, good.get(), w, single.get(), inpair.get());
     BCMGX_LAUNCHED();
-    int nsingles = scan_exclusive(c, single.get(), sid.get(), n);
-    k_assign_singles<<<grid_for(n, WG), WG, 0, c->stream>>>(n, npairs, inpair.get(), sid.get(), w, agg, pval);
+    scan_exclusive_async(c, single.get(), sid.get(), n);
+    k_assign_singles<<<grid_for(n, WG), WG, 0, c->stream>>>(n, pid.get() + n, inpair.get(), sid.get(), w, agg, pval);
     BCMGX_LAUNCHED();
+    int npairs, nsingles;
+    c->d2h2(&npairs, pid.get() + n, 1, &nsingles, sid.get() + n, 1);
     return npairs + nsingles;
 }
 
@@ -600,7 +603,7 @@ static void transpose_raw(Context* c, int n, int m, long long nnz, const int* ro
         k_count_cols<<<grid_for(nnz, WG), WG, 0, c->stream>>>(nnz, col, cnt.get());
         BCMGX_LAUNCHED();
     }
-    scan_exclusive(c, cnt.get(), T.row.get(), m);
+    scan_exclusive_async(c, cnt.get(), T.row.get(), m);
     if (n > 0) {
         k_scatter_T<<<grid_for(n, WG), WG, 0, c->stream>>>(n, row, col, val, T.row.get(), cursor.get(), T.col.get(), T.val.get());
         BCMGX_LAUNCHED();
@@ -947,17 +950,18 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     DBuf<long long> cnt(c, n), poff(c, (size_t)n + 1);
     k_count_products<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
     BCMGX_LAUNCHED();
-    long long total = scan_exclusive(c, cnt.get(), poff.get(), n);
-    size_t tsz = (size_t)std::max<long long>(total, 1);
-    DBuf<int> tcol(c, tsz);
-    DBuf<double> tval(c, tsz);
-    DBuf<int> uniq(c, (size_t)n + 1);
+    scan_exclusive_async(c, cnt.get(), poff.get(), n);
     DBuf<int> bins(c, (size_t)NBINS * n), bincount(c, NBINS);
     fill<int>(c, bincount.get(), NBINS, 0);
     k_bin_rows<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), cnt.get(), bins.get(), bincount.get(), n);
     BCMGX_LAUNCHED();
+    long long total;
     int bc[NBINS];
-    c->d2h(bc, bincount.get(), NBINS);
+    c->d2h2(&total, poff.get() + n, 1, bc, bincount.get(), NBINS);  // product count and bin sizes, one round trip
+    size_t tsz = (size_t)std::max<long long>(total, 1);
+    DBuf<int> tcol(c, tsz);
+    DBuf<double> tval(c, tsz);
+    DBuf<int> uniq(c, (size_t)n + 1);
 #define BCMGX_SMALL(BIN, W)                                                                                                 \
     if (bc[BIN] > 0) {                                                                                                      \
         k_spgemm_small<W><<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(                                                 \
@@ -989,7 +993,8 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
                                                  B.row.get(), B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(),
                                                  uniq.get(), gk.get(), gv.get(), gp.get(), soff.get(), aoff.get());
         BCMGX_LAUNCHED();
-        c->sync();  // keep the scratch alive until the kernel is done
+        // the scratch goes back to the pool here; the pool hands blocks out again only to
+        // work queued behind this kernel on the same stream
     }
     C.row.reset(c, (size_t)n + 1);
     long long nnz = scan_exclusive(c, uniq.get(), C.row.get(), n);
@@ -1000,7 +1005,6 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     k_spgemm_compact<<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff.get(), C.row.get(), tcol.get(), tval.get(), C.col.get(),
                                                                 C.val.get());
     BCMGX_LAUNCHED();
-    c->sync();
 }
 
 // ------------------------------------------------------ adjoint gradient ---
