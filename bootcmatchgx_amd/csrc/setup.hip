@@ -347,8 +347,11 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, c
     // (the launch is throughput-bound), longer once only chains remain (each
     // launch is then latency-bound and fewer launches finish them sooner)
     auto maxsteps_of = [](int round) { return round < 8 ? 8 : round < 16 ? 16 : round < 24 ? 32 : 64; };
-    DBuf<int> la(c, n), lb(c, n), cnt(c, 2);
-    fill<int>(c, cnt.get(), 2, 0);
+    // round r counts its pending vertices in cnt[r % RING]: a ring of counters
+    // cleared together every RING rounds, not one clear per round
+    constexpr int RING = 64;
+    DBuf<int> la(c, n), lb(c, n), cnt(c, RING);
+    fill<int>(c, cnt.get(), RING, 0);
     unsigned grid = (unsigned)std::min<long long>(grid_for(n, WG / G), (long long)c->num_cus * 16);
     k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, nullptr, nullptr, la.get(), cnt.get(),
                                              maxsteps_of(0));
@@ -356,13 +359,15 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, c
     int* lists[2] = {la.get(), lb.get()};
     int cur = 0;
     for (int round = 1;; round++) {
-        // lists[cur] / cnt[cur] hold the pending vertices; clear the other counter and run a round
-        fill<int>(c, cnt.get() + (cur ^ 1), 1, 0);
-        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, lists[cur], cnt.get() + cur,
-                                                 lists[cur ^ 1], cnt.get() + (cur ^ 1), maxsteps_of(round));
+        // lists[cur] holds the pending vertices, counted in cnt[(round - 1) % RING]
+        if (round % RING == 0) fill<int>(c, cnt.get(), RING - 1, 0);  // every counter but this round's input
+        const int* in_cnt = cnt.get() + (round - 1) % RING;
+        int* out_cnt = cnt.get() + round % RING;
+        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, lists[cur], in_cnt, lists[cur ^ 1],
+                                                 out_cnt, maxsteps_of(round));
         BCMGX_LAUNCHED();
         cur ^= 1;
-        if (round % 8 == 0 && c->read1(cnt.get() + cur) == 0) break;
+        if (round % 8 == 0 && c->read1(out_cnt) == 0) break;
         // every round settles at least one proposal; offers are bounded by the edge count
         BCMGX_CHECK(round < 4 * n + 1024, BCMGX_ERR_NUMERIC, "matching did not settle");
     }
