@@ -62,6 +62,7 @@ class Params(C.Structure):
         ("use_graph", C.c_int),
         ("autotune", C.c_int),
         ("check_every", C.c_int),
+        ("fcg_z32", C.c_int),
     ]
 
 
@@ -124,4 +125,5 @@ def reference_defaults():
     p.use_graph = 1
     p.autotune = 1
     p.check_every = 8
+    p.fcg_z32 = 1
     return p

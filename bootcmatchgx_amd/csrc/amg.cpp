@@ -261,6 +261,10 @@ struct SolveWork {
     FcgBufs fb;
     FcgEntry entry;     // level-0 entry of the FP32 cycle filled by the updates
     bool fused = false;
+    // FP32 z: the cycle's result kept in FP32 and widened inside the fused product and the update
+    bool z32 = false;
+    DBuf<float> zf;
+    int z32_blocks = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t cg_graph = nullptr;  // one CG iteration
@@ -473,6 +477,15 @@ bool Prec::fused_entry(int k, FcgEntry* e) {
     return true;
 }
 
+void Prec::apply_entry_f32(float* z, int k, const int* stop) {
+    BCMGX_CHECK(ptype == BCMGX_PREC_BCMG && p.precision == BCMGX_PRECISION_FP32 && H.levels() >= 2 && p.prerelax_sweeps >= 1,
+                BCMGX_ERR_STATE, "level-0 entry not applicable");
+    ensure_work(k);
+    CycleRun<float> R{ctx, this, &wf, k, stop};
+    float* res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, nullptr, z);
+    if (res != z) copy_d2d(ctx, z, res, sizeof(float) * (size_t)A->n * k);
+}
+
 void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready) {
     ensure_work(k);
     Context* c = ctx;
@@ -580,7 +593,27 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
 
 // ---------------------------------------------------------------- FCG ----
 
+// The same iteration with z in FP32: the direction d and its image q are updated
+// in place (no ping-pong), w holds A z.  The same operations in the same order as
+// the widened-FP64 iteration; only the dot products group their sums differently.
+static void fcg_iteration_z32(Context* c, const DCsr& A, Prec* prec, SolveWork& s, int idx) {
+    const int* stop = s.flags.get();
+    prec->apply_entry_f32(s.zf.get(), s.k, stop);
+    SpmvArgs a;
+    a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+    a.x = s.zf.get(); a.y = s.w.get(); a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
+    BCMGX_CHECK(spmm_dots_z32(c, a, s.r.get(), s.q.get(), s.fb.partials, s.z32_blocks, s.fb.all_done), BCMGX_ERR_STATE,
+                "fused FP32-z product not applicable");
+    fcg_scalars3(c, s.fb, s.z32_blocks, idx);
+    fcg_update_z32(c, s.fb, s.zf.get(), s.d.get(), s.w.get(), s.q.get(), s.X.get(), s.r.get(), idx, s.entry);
+    fcg_finish_iteration(c, s.fb, idx);
+}
+
 static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, int idx) {
+    if (s.z32) {
+        fcg_iteration_z32(c, A, prec, s, idx);
+        return;
+    }
     double* P0 = idx == 0 ? s.v.get() : s.d.get();
     double* W0 = idx == 0 ? s.w.get() : s.q.get();
     double* P1 = idx == 0 ? s.d.get() : s.v.get();
@@ -619,7 +652,8 @@ static SolveWork& solve_work(Context* c, const DCsr& A, Prec* prec, const bcmgx_
         s.fb.k = k;
         s.fb.nblk = fcg_partial_blocks(n);
         s.mv_blocks = spmm_dots_blocks(c, n, k);
-        s.partials.reset(c, (size_t)std::max(s.fb.nblk, s.mv_blocks) * k * 3);
+        s.z32_blocks = spmm_dots_z32_blocks(c, n, k);
+        s.partials.reset(c, (size_t)std::max({s.fb.nblk, s.mv_blocks, s.z32_blocks}) * k * 3);
         s.st.reset(c, k);
         s.flags.reset(c, 2);
         s.fb.partials = s.partials.get();
@@ -642,6 +676,11 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     prec->ensure_work(k);
     s.entry = FcgEntry();
     s.fused = prec->fused_entry(k, &s.entry);
+    // z in FP32 when the FP32 cycle is entered from the update and the fused FP32-z product applies
+    bool z32 = s.fused && p.fcg_z32 && s.z32_blocks > 0 && s.z32_blocks % 8 == 0 && (s.chunkA <= 0 || s.chunkA >= k);
+    if (z32 != s.z32) s.drop_graph();  // the captured iterations differ
+    s.z32 = z32;
+    if (s.z32 && !s.zf) s.zf.reset(c, (size_t)std::max(n, 1) * k);
     long long nk = (long long)n * k;
     copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
     copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
@@ -697,8 +736,8 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         if (launched > p.itnlim + 2) break;  // the device stops every column at itnlim
     }
     // every column stopped after the first iteration of the last pair: its x step waits in pend
-    // and its direction is still in v (the first iteration's p0)
-    fcg_flush_pending(c, s.fb, s.v.get(), s.X.get());
+    // and its direction is still in v (the first iteration's p0), or in d with z in FP32
+    fcg_flush_pending(c, s.fb, s.z32 ? s.d.get() : s.v.get(), s.X.get());
     copy_d2d(c, x, s.X.get(), sizeof(double) * nk);
     std::vector<FcgState> st(k);
     c->d2h(st.data(), s.st.get(), k);

@@ -73,6 +73,9 @@ struct Prec {
     // the level-0 buffers an FCG update can fill for the next apply (FP32 BCMG
     // with at least two levels and a pre-smoothing sweep); false otherwise
     bool fused_entry(int k, FcgEntry* e);
+    // the FP32 cycle from that entry, its result z left in FP32 (k interleaved
+    // columns): the values the FP64 output would hold, before the exact widening
+    void apply_entry_f32(float* z, int k, const int* stop);
 };
 
 std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& p);

@@ -286,6 +286,70 @@ void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, doub
     BCMGX_LAUNCHED();
 }
 
+// k_update with z in FP32 (widened exactly) and p, w updated in place
+template <int PHASE>
+__global__ __launch_bounds__(WG) void k_update_z32(int n, int k, int KP, const FcgState* __restrict__ st,
+                                                   const float* __restrict__ z, double* __restrict__ p,
+                                                   const double* __restrict__ w0, double* __restrict__ w,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ partials, const int* __restrict__ all_done,
+                                                   FcgEntry e) {
+    if (*all_done) return;
+    __shared__ double s[WG];
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    double acc = 0.0;
+    if (c < k) {
+        const FcgState S = st[c];
+        double mth = -S.theta, th2 = S.theta2, mth2 = -S.theta2;
+        bool waiting = PHASE == 1 && S.pad1;
+        double pend = S.pend;
+        for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+            long long o = i * k + c;
+            double rr;
+            if (!S.done) {
+                double po = p[o];
+                double t = mth * po + (double)z[o];
+                if constexpr (PHASE == 1) {
+                    double xo = x[o];
+                    double xw = pend * po + xo;
+                    x[o] = th2 * t + (waiting ? xw : xo);
+                }
+                p[o] = t;
+                double u = mth * w[o] + w0[o];
+                rr = mth2 * u + r[o];
+                r[o] = rr;
+                w[o] = u;
+            } else {
+                rr = r[o];
+                if (waiting) x[o] = pend * p[o] + x[o];
+            }
+            acc += rr * rr;
+            if (e.F) {
+                float f = (float)rr;
+                e.F[o] = f;
+                e.X[o] = f / e.M[i];
+            }
+        }
+    }
+    s[tid] = acc;
+    __syncthreads();
+    for (int stp = WG / 2; stp >= KP; stp >>= 1) {
+        if (tid < stp) s[tid] += s[tid + stp];
+        __syncthreads();
+    }
+    if (tid < k) partials[((long long)blockIdx.x * k + tid) * 3] = s[tid];
+}
+
+void fcg_update_z32(Context* c, const FcgBufs& f, const float* z, double* p, const double* w0, double* w, double* x, double* r,
+                    int phase, const FcgEntry& e) {
+    int KP = kpad(f.k);
+    if (phase == 0) k_update_z32<0><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, z, p, w0, w, x, r, f.partials, f.all_done, e);
+    else k_update_z32<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, z, p, w0, w, x, r, f.partials, f.all_done, e);
+    BCMGX_LAUNCHED();
+}
+
 // l2 = ||r||; iter++; stop when !(l2 > rtol*delta0 && iter < itnlim)  (FCG.cu:146-156)
 __global__ __launch_bounds__(WG) void k_finish(const double* __restrict__ partials, int nblk, int k, FcgState* st,
                                                double rtol, int itnlim, int* all_done, int* ndone, int phase) {

@@ -79,6 +79,12 @@ void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const 
 template <typename A, typename B>
 void convert_stop(Context* c, const A* s, B* d, long long n, const int* stop);
 
+// the same with z (the SpMM's x) in FP32, widened exactly on load (four columns
+// per lane); 0 / false unless k is a multiple of 4 in [8, 64]
+int spmm_dots_z32_blocks(Context* c, int n, int k);
+bool spmm_dots_z32(Context* c, const SpmvArgs& a, const double* r, const double* wold, double* partials, int nblk,
+                   const int* all_done);
+
 // ---- FCG kernels (fcg.hip) ----------------------------------------------------
 struct FcgState {
     double delta0, delta, l2, alpha, beta, gamma, theta, theta2;
@@ -121,6 +127,11 @@ void fcg_update(Context* c, const FcgBufs& f, double* p0, const double* p1, doub
                 int phase, const FcgEntry& e = FcgEntry());
 // ||r|| -> convergence test and iteration count (FCG.cu:146-156)
 void fcg_finish_iteration(Context* c, const FcgBufs& f, int phase);
+// the update with the preconditioned residual z in FP32 and the direction p and
+// its image w updated in place: t = z - theta p, u = w0 - theta w; p = t, w = u
+// (the same operations, in the same order, as fcg_update)
+void fcg_update_z32(Context* c, const FcgBufs& f, const float* z, double* p, const double* w0, double* w, double* x, double* r,
+                    int phase, const FcgEntry& e);
 // a step still waiting after the last pair (every column stopped after its
 // first iteration): x += pend t, t the first iteration's direction
 void fcg_flush_pending(Context* c, const FcgBufs& f, const double* t, double* x);

@@ -103,6 +103,7 @@ int bcmgx_params_default(bcmgx_params* p) {
     p->use_graph = 1;
     p->autotune = 1;
     p->check_every = 8;
+    p->fcg_z32 = 1;
     return BCMGX_OK;
 }
 
@@ -167,6 +168,7 @@ int bcmgx_params_set(bcmgx_params* p, const char* key_c, const char* value_c) {
     if (key == "use_graph") return seti(&p->use_graph);
     if (key == "autotune") return seti(&p->autotune);
     if (key == "check_every") return seti(&p->check_every);
+    if (key == "fcg_z32") return seti(&p->fcg_z32);
     return BCMGX_OK;  // unknown keys are ignored (Params.cu:345-398)
 }
 

@@ -204,17 +204,19 @@ struct SpmmExtra {
 //   MODE 2 (FP64, AX): w = A p and FCG's triple dot product (r.p, w.p, wold.p)
 //          per column (triple_innerproduct, src/op/triple_inner_product.cu:7-101),
 //          reduced per block in a fixed order into ex.partials[(blk*k + c)*3 + m].
-template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT = false>
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT = false, typename XT = V>
 __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
-                                              const T* __restrict__ val, const V* __restrict__ x, V* __restrict__ y,
+                                              const T* __restrict__ val, const XT* __restrict__ x, V* __restrict__ y,
                                               const V* __restrict__ b, const V* __restrict__ D, V alpha, V beta,
                                               const int* __restrict__ stop, SpmmExtra ex, int sg) {
     typedef V VV __attribute__((ext_vector_type(CPL)));
+    typedef XT XV __attribute__((ext_vector_type(CPL)));  // x may be narrower than V: widened exactly on load
     typedef double VD __attribute__((ext_vector_type(CPL)));
     if (stop && *stop) return;
     constexpr int RPB = WG / KP;
     constexpr int E = KP * M;
-    constexpr int U = 16;
+    // gathers in flight per lane: 16, or 8 for widened 4-column FP64 accumulators (registers)
+    constexpr int U = (sizeof(XT) < sizeof(V) && CPL >= 4) ? 8 : 16;
     __shared__ int scol[WG * M];
     __shared__ T sval[WG * M];
     int tid = threadIdx.x;
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
     int g0 = (int)((long long)ngroups * xcd / X), g1 = (int)((long long)ngroups * (xcd + 1) / X);
     int* sc = scol + g * E;
     T* sv = sval + g * E;
-    const V* xc = x + cl * CPL;
+    const XT* xc = x + cl * CPL;
     VD da = (VD)0.0, db = (VD)0.0, dc = (VD)0.0;
     // the t-th row group of this XCD, -1 past its last (increasing in t)
     auto group_of = [&](int q) -> int {
@@ -264,15 +266,27 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
     auto gather = [&](int cnt, VV& t) {
         if (!active) return;
         for (int q = 0; q < cnt; q += U) {
-            VV xv[U];
+            XV xv[U];
+            if constexpr (std::is_same<XT, V>::value) {
 #pragma unroll
-            for (int u = 0; u < U; u++) xv[u] = q + u < cnt ? *(const VV*)(xc + (long long)sc[q + u] * k) : (VV)(V)0;
+                for (int u = 0; u < U; u++) xv[u] = q + u < cnt ? *(const XV*)(xc + (long long)sc[q + u] * k) : (XV)(XT)0;
+            } else {
+                // narrow x: unconditional loads (past the row, row 0, weighted by nothing),
+                // so that the widening stays out of the load branches and all U loads are
+                // in flight together
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    int cc = q + u < cnt ? sc[q + u] : 0;
+                    xv[u] = *(const XV*)(xc + (long long)cc * k);
+                }
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (q + u < cnt) {
                     V a = (V)sv[q + u];
-                    if (OP == SpOp::AXPBY) t += (alpha * a) * xv[u];
-                    else t += a * xv[u];
+                    VV xw = __builtin_convertvector(xv[u], VV);
+                    if (OP == SpOp::AXPBY) t += (alpha * a) * xw;
+                    else t += a * xw;
                 }
             }
         }
@@ -317,11 +331,11 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
         if (ri >= 0 && active) {
             long long o = (long long)ri * k + cl * CPL;
             if constexpr (MODE == 1) {
-                VV res = ((-t + *(const VV*)(b + o)) / D[ri]) + *(const VV*)(x + o);
+                VV res = ((-t + *(const VV*)(b + o)) / D[ri]) + __builtin_convertvector(*(const XV*)(x + o), VV);
                 *(VD*)(ex.y64 + o) = __builtin_convertvector(res, VD);
             } else if constexpr (MODE == 2) {
                 *(VV*)(y + o) = t;
-                VD pw = *(const VD*)(x + o);
+                VD pw = __builtin_convertvector(*(const XV*)(x + o), VD);
                 da += *(const VD*)(ex.r + o) * pw;
                 db += t * pw;
                 dc += *(const VD*)(ex.wold + o) * pw;
@@ -330,7 +344,7 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
                 if (OP == SpOp::AX) *yo = t;
                 else if (OP == SpOp::RES) *yo = *(const VV*)(b + o) - t;
                 else if (OP == SpOp::AXPBY) *yo = t + beta * *yo;
-                else *yo = ((-t + *(const VV*)(b + o)) / D[ri]) + *(const VV*)(x + o);
+                else *yo = ((-t + *(const VV*)(b + o)) / D[ri]) + __builtin_convertvector(*(const XV*)(x + o), VV);
             }
         }
         it += nloc;
@@ -386,15 +400,15 @@ static int persistent_grid(Context* c, K kernel) {
 // 2^(-16 - kchunk) groups; other choices keep the per-XCD segments (0)
 static int kchunk_sg(int kchunk) { return kchunk <= -16 && kchunk >= -28 ? 1 << (-16 - kchunk) : 0; }
 
-template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT>
+template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT, typename XT = V>
 static void launch_spmmp(Context* c, const SpmvArgs& a, const SpmmExtra& ex, int grid) {
-    auto kern = k_spmmp<T, V, KP, CPL, M, OP, MODE, NT>;
+    auto kern = k_spmmp<T, V, KP, CPL, M, OP, MODE, NT, XT>;
     if (grid <= 0) {
         static int cached = 0;  // per instantiation; identical devices assumed within a process
         if (!cached) cached = persistent_grid(c, kern);
         grid = cached;
     }
-    kern<<<grid, WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, (const T*)a.val, (const V*)a.x, (V*)a.y, (const V*)a.b,
+    kern<<<grid, WG, 0, c->stream>>>(a.n, a.k, a.row, a.col, (const T*)a.val, (const XT*)a.x, (V*)a.y, (const V*)a.b,
                                      (const V*)a.D, (V)a.alpha, (V)a.beta, a.stop, ex, kchunk_sg(a.kchunk));
     BCMGX_LAUNCHED();
 }
@@ -452,6 +466,47 @@ bool spmm_dots(Context* c, const SpmvArgs& a, const double* r, const double* wol
     b.stop = all_done;
     if (a.kchunk == -2) return dispatch_spmmp<double, double, SpOp::AX, 2, true>(c, b, ex, nblk);
     return dispatch_spmmp<double, double, SpOp::AX, 2>(c, b, ex, nblk);
+}
+
+// The fused FCG product with the preconditioned residual z held in FP32 -- the
+// values an FP32 V-cycle computes, before any widening.  Each lane gathers four
+// columns as one 16-byte float4 and widens them exactly into FP64 accumulators:
+// W0 has the bits of the widened-FP64 path (each column's row sum in order), from
+// half the gathered bytes and half the load instructions per row; the three dot
+// products are summed over this kernel's own (fixed) row groups.
+template <int KP, int M>
+static int z32_grid(Context* c) {
+    return persistent_grid(c, k_spmmp<double, double, KP, 4, M, SpOp::AX, 2, false, float>);
+}
+
+int spmm_dots_z32_blocks(Context* c, int n, int k) {
+    (void)n;
+    switch (spmmp_lanes(k, 4)) {
+    case 2: return z32_grid<2, 8>(c);
+    case 4: return z32_grid<4, 8>(c);
+    case 8: return z32_grid<8, 4>(c);
+    case 16: return z32_grid<16, 2>(c);
+    default: return 0;
+    }
+}
+
+bool spmm_dots_z32(Context* c, const SpmvArgs& a, const double* r, const double* wold, double* partials, int nblk,
+                   const int* all_done) {
+    bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)r | (uintptr_t)wold) & 15) == 0;
+    if (nblk <= 0 || nblk % 8 || !aligned || (a.kchunk > 0 && a.kchunk < a.k)) return false;
+    SpmmExtra ex;
+    ex.r = r;
+    ex.wold = wold;
+    ex.partials = partials;
+    SpmvArgs b = a;
+    b.stop = all_done;
+    switch (spmmp_lanes(a.k, 4)) {
+    case 2: launch_spmmp<double, double, 2, 4, 8, SpOp::AX, 2, false, float>(c, b, ex, nblk); return true;
+    case 4: launch_spmmp<double, double, 4, 4, 8, SpOp::AX, 2, false, float>(c, b, ex, nblk); return true;
+    case 8: launch_spmmp<double, double, 8, 4, 4, SpOp::AX, 2, false, float>(c, b, ex, nblk); return true;
+    case 16: launch_spmmp<double, double, 16, 4, 2, SpOp::AX, 2, false, float>(c, b, ex, nblk); return true;
+    default: return false;
+    }
 }
 
 // CSR-stream SpMV for one vector: a workgroup owns RB = 256/TPR consecutive

@@ -88,6 +88,8 @@ typedef struct bcmgx_params {
     int use_graph;          /* "use_graph": capture the FCG iteration in a hipGraph (default 1) */
     int autotune;           /* "autotune": pick SpMV/SpMM lanes-per-row by timing (default 1) */
     int check_every;        /* "check_every": FCG iterations between host convergence polls (default 8) */
+    int fcg_z32;            /* "fcg_z32": with an FP32 hierarchy, FCG keeps the cycle's result z in FP32 and
+                               widens it inside its kernels (same results, fewer bytes; default 1) */
 } bcmgx_params;
 
 /* Defaults of src/config/Params.h:238-291. */

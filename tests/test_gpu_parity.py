@@ -458,6 +458,33 @@ def test_missing_diagonal_is_reported(ctx):
     assert "row 7 has no diagonal" in str(e.value)
 
 
+@pytest.mark.parametrize("k", [4, 8, 32, 12, 64])
+def test_fcg_fp32_z_matches_widened_path(ctx, k):
+    """With an FP32 hierarchy FCG keeps the cycle's result in FP32 (fcg_z32) and widens it inside
+    its kernels: the same operations as the widened-FP64 path, the dot products summed in other
+    (fixed) groups; iteration counts within 1, solutions within 1e-10, both meet the tolerance."""
+    oa = O.laplacian3d(27, 20, 20, 20)
+    n = oa.dims()[0]
+    A = dev(ctx, oa)
+    B = np.random.default_rng(29).normal(size=(n, k))
+    res = {}
+    for z in (1, 0):
+        p = params("BCMG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, precision="FP32", fcg_z32=z)
+        P = bx.prec_setup(ctx, A, p)
+        X, out = bx.solve(ctx, A, B, None, p, P)
+        X2, out2 = bx.solve(ctx, A, B, None, p, P)  # the replayed graph
+        assert np.array_equal(X, X2)
+        res[z] = (X, out)
+        P.finalize()
+    (X1, o1), (X0, o0) = res[1], res[0]
+    assert o1["retv"] == o0["retv"] == 0
+    for j in range(k):
+        assert abs(o1["niter_cols"][j] - o0["niter_cols"][j]) <= 1
+        assert np.linalg.norm(X1[:, j] - X0[:, j]) <= 1e-10 * np.linalg.norm(X0[:, j])
+        r = B[:, j] - O.spmv(oa, X1[:, j])
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, j]) * 1.0000001
+
+
 def test_itnlim_and_info_file(ctx, tmp_path):
     oa = O.laplacian3d(7, 12, 12, 12)
     A = dev(ctx, oa)
