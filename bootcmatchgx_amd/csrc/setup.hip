@@ -191,11 +191,11 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
 
 // Preference ranks.  Vertex y orders its neighbours u by (W(y,u), u) -- the
 // Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
-// rank[e] = position of entry e of row y in that order; inv[Wrow[y] + r] = u.
+// rank[e] = position of entry e of row y in that order (0: the least liked).
 // G lanes per row, each ranking a strided subset of the row's entries.
 template <int G>
 __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                             const double* __restrict__ Wval, int* __restrict__ rank, int* __restrict__ inv) {
+                                             const double* __restrict__ Wval, int* __restrict__ rank) {
     int y = (blockIdx.x * WG + (int)threadIdx.x) / G;
     int lane = (int)threadIdx.x % G;
     if (y >= n) return;
@@ -212,10 +212,7 @@ __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow
             int cb = __shfl(ca, b, G);
             r += (b < len && (wb < wa || (wb == wa && cb < ca))) ? 1 : 0;
         }
-        if (va) {
-            rank[s + lane] = r;
-            inv[s + r] = ca;
-        }
+        if (va) rank[s + lane] = r;
         return;
     }
     for (int a = s + lane; a < e; a += G) {
@@ -227,48 +224,65 @@ __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow
             r += (wb < wa || (wb == wa && Wcol[b] < ca)) ? 1 : 0;
         }
         rank[a] = r;
-        inv[s + r] = ca;
     }
 }
 
-// tkey[e] for entry e = (u, y) of row u: 1 + rank of u among y's neighbours
+// Preference lists.  pref[Wrow[u] + j] = {y, key} for the j-th best neighbour
+// y of u in u's own order (the reverse of rank: heaviest first, ties to the
+// higher id), with key = 1 + the rank of u among y's neighbours.  key = 0 marks
+// a neighbour u never proposes to: non-positive weight (find_Candidate,
+// suitor.cu:91-94) or no mirrored entry (u is unknown to y).
 template <int G>
-__global__ __launch_bounds__(WG) void k_tkey(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                             const int* __restrict__ rank, int* __restrict__ tkey) {
+__global__ __launch_bounds__(WG) void k_pref(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
+                                             const double* __restrict__ Wval, const int* __restrict__ rank,
+                                             int2* __restrict__ pref) {
     int u = (blockIdx.x * WG + (int)threadIdx.x) / G;
     int lane = (int)threadIdx.x % G;
     if (u >= n) return;
-    for (int e = Wrow[u] + lane; e < Wrow[u + 1]; e += G) {
-        int y = Wcol[e];
-        int b = Wrow[y], len = Wrow[y + 1] - b;
-        int p = lower_bound_dev(Wcol + b, len, u);
-        // no mirrored entry: key 0 never beats an offer, u cannot propose to y
-        tkey[e] = (p < len && Wcol[b + p] == u) ? rank[b + p] + 1 : 0;
+    int s = Wrow[u], e = Wrow[u + 1];
+    for (int a = s + lane; a < e; a += G) {
+        int y = Wcol[a];
+        int key = 0;
+        if (Wval[a] > 0 && y >= 0 && y < n) {
+            int b = Wrow[y], len = Wrow[y + 1] - b;
+            int p = lower_bound_dev(Wcol + b, len, u);
+            if (p < len && Wcol[b + p] == u) key = rank[b + p] + 1;
+        }
+        pref[e - 1 - rank[a]] = make_int2(y, key);
     }
 }
 
-// Lock-free Suitor.  S[y] holds 1 + the rank of y's current suitor (0: none).
-// A vertex proposes to its best neighbour -- positive weight, heaviest, ties
-// to the higher id (find_Candidate, suitor.cu:80-147) -- that would prefer it
-// to its current suitor; atomicMax installs the offer and returns the
-// displaced suitor, which this thread then re-proposes (set_suitor,
-// suitor.cu:51-78, without the spin locks).  Every failed atomicMax means
-// another offer raised S[best], and offers only rise, so the grid drains.
-// The fixed point is the unique stable (greedy) matching of the total order
-// (W, max id, min id), whatever the interleaving.
-// One round of the lock-free Suitor.  Each G-lane group takes a vertex from
-// the round's work list (all vertices in round 0) and follows its proposal
-// chain for at most `maxsteps` proposals; a vertex still unsettled then goes
-// to the next round's list.  Bounding the steps keeps one long displacement
-// chain from serialising a whole launch: every round works on all pending
-// vertices in parallel with fresh offers.
+// lanes per proposing vertex (the candidates one ballot looks at)
+constexpr int SUITOR_LANES = 16;
+
+// Lock-free Suitor.  S[y] = (1 + rank of y's current suitor) << 32 | suitor
+// (0: none).  A vertex proposes to its best neighbour -- positive weight,
+// heaviest, ties to the higher id (find_Candidate, suitor.cu:80-147) -- that
+// would prefer it to its current suitor; atomicMax installs the offer and
+// returns the displaced suitor, which this thread then re-proposes
+// (set_suitor, suitor.cu:51-78, without the spin locks).  Every failed
+// atomicMax means another offer raised S[y], and offers only rise, so the grid
+// drains.  The fixed point is the unique stable (greedy) matching of the total
+// order (W, max id, min id), whatever the interleaving.
+//
+// One launch is one round.  Each G-lane group takes a vertex from the round's
+// work list (all vertices in round 0) and follows its proposal chain for at
+// most `maxsteps` proposals; a vertex still unsettled then goes to the next
+// round's list.  Bounding the steps keeps one long displacement chain from
+// serialising a whole launch: every round works on all pending vertices in
+// parallel with fresh offers.  A proposal step is a chain of dependent loads
+// (row bounds, preference list, suitor words, atomicMax), so it is latency-
+// bound: the preference list in order lets the group take the first eligible
+// neighbour of its first G (one ballot, no weight comparisons), and the suitor
+// word names the displaced vertex without a lookup.
 template <int G>
-__global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                               const double* __restrict__ Wval, const int* __restrict__ tkey,
-                                               const int* __restrict__ inv, int* S, const int* __restrict__ in_list,
+__global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wrow, const int2* __restrict__ pref,
+                                               unsigned long long* S, const int* __restrict__ in_list,
                                                const int* __restrict__ in_count, int* __restrict__ out_list,
                                                int* __restrict__ out_count, int maxsteps) {
+    static_assert(G <= 32, "group mask");
     int lane = (int)threadIdx.x % G;
+    int gbase = ((int)threadIdx.x & 63) - lane;  // the group's first lane in its wave
     int ngroups = gridDim.x * (WG / G);
     int total = in_list ? *in_count : n;
     for (int it = (blockIdx.x * WG + (int)threadIdx.x) / G; it < total; it += ngroups) {
@@ -282,49 +296,46 @@ __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wr
                 break;
             }
             steps++;
+            int s = Wrow[cur], e = Wrow[cur + 1];
             int best = -1, bkey = 0;
-            double bw = 0.0;
-            for (int e = Wrow[cur] + lane; e < Wrow[cur + 1]; e += G) {
-                double wv = Wval[e];
-                if (!(wv > 0)) continue;
-                int y = Wcol[e];
-                if (y < 0 || y >= n) continue;
-                int k = tkey[e];
-                int sy = __hip_atomic_load(&S[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (sy >= k) continue;
-                if (best < 0 || wv > bw || (wv == bw && y > best)) {
-                    best = y;
-                    bw = wv;
-                    bkey = k;
+            for (int b = s; b < e; b += G) {  // G candidates at a time, best first
+                int2 pk = make_int2(-1, 0);
+                if (b + lane < e) pk = pref[b + lane];
+                bool ok = false;
+                // (the range test only matters for rows with repeated columns,
+                // which leave slots of the list unwritten)
+                if (pk.y > 0 && (unsigned)pk.x < (unsigned)n) {
+                    unsigned long long sy = __hip_atomic_load(&S[pk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = (int)(sy >> 32) < pk.y;
                 }
-            }
-            // heaviest candidate of the group, ties to the higher id (find_Candidate's reduction)
-#pragma unroll
-            for (int o = G / 2; o > 0; o >>= 1) {
-                double pw = __shfl_xor(bw, o, G);
-                int pb = __shfl_xor(best, o, G);
-                int pk = __shfl_xor(bkey, o, G);
-                if (pb >= 0 && (best < 0 || pw > bw || (pw == bw && pb > best))) {
-                    best = pb;
-                    bw = pw;
-                    bkey = pk;
+                unsigned long long m = (__ballot(ok) >> gbase) & ((1ull << G) - 1);
+                if (m) {
+                    int f = __ffsll(m) - 1;
+                    best = __shfl(pk.x, f, G);
+                    bkey = __shfl(pk.y, f, G);
+                    break;
                 }
             }
             if (best < 0) break;
-            int old = 0;
-            if (lane == 0) old = atomicMax(&S[best], bkey);
-            old = __shfl(old, 0, G);
-            if (old < bkey) cur = old > 0 ? inv[Wrow[best] + old - 1] : -1;
+            unsigned olo = 0, ohi = 0;
+            if (lane == 0) {
+                unsigned long long offer = ((unsigned long long)(unsigned)bkey << 32) | (unsigned)cur;
+                unsigned long long old = atomicMax(&S[best], offer);
+                olo = (unsigned)old;
+                ohi = (unsigned)(old >> 32);
+            }
+            olo = __shfl(olo, 0, G);
+            ohi = __shfl(ohi, 0, G);
+            if ((int)ohi < bkey) cur = ohi > 0 ? (int)olo : -1;
         }
     }
 }
 
-__global__ void k_suitor_decode(int n, const int* __restrict__ Wrow, const int* __restrict__ inv, const int* __restrict__ S,
-                                int* __restrict__ M) {
+__global__ void k_suitor_decode(int n, const unsigned long long* __restrict__ S, int* __restrict__ M) {
     int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= n) return;
-    int s = S[y];
-    M[y] = s > 0 ? inv[Wrow[y] + s - 1] : -1;
+    unsigned long long s = S[y];
+    M[y] = (s >> 32) ? (int)(unsigned)s : -1;
 }
 
 
@@ -341,8 +352,7 @@ __global__ void k_make_wrow(int n, const int* __restrict__ row, int* __restrict_
 // Rounds of k_suitor until no vertex is pending.  The list length is read
 // back every few rounds (an empty round costs one tiny launch).
 template <int G>
-static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, const double* Wval, const int* tkey,
-                          const int* inv, int* S) {
+static void suitor_rounds(Context* c, int n, const int* Wrow, const int2* pref, unsigned long long* S) {
     // proposals per vertex per launch: short while most vertices are pending
     // (the launch is throughput-bound), longer once only chains remain (each
     // launch is then latency-bound and fewer launches finish them sooner)
@@ -353,7 +363,7 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, c
     DBuf<int> la(c, n), lb(c, n), cnt(c, RING);
     fill<int>(c, cnt.get(), RING, 0);
     unsigned grid = (unsigned)std::min<long long>(grid_for(n, WG / G), (long long)c->num_cus * 16);
-    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, nullptr, nullptr, la.get(), cnt.get(),
+    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, pref, S, nullptr, nullptr, la.get(), cnt.get(),
                                              maxsteps_of(0));
     BCMGX_LAUNCHED();
     int* lists[2] = {la.get(), lb.get()};
@@ -363,7 +373,7 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int* Wcol, c
         if (round % RING == 0) fill<int>(c, cnt.get(), RING - 1, 0);  // every counter but this round's input
         const int* in_cnt = cnt.get() + (round - 1) % RING;
         int* out_cnt = cnt.get() + round % RING;
-        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, Wcol, Wval, tkey, inv, S, lists[cur], in_cnt, lists[cur ^ 1],
+        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, pref, S, lists[cur], in_cnt, lists[cur ^ 1],
                                                  out_cnt, maxsteps_of(round));
         BCMGX_LAUNCHED();
         cur ^= 1;
@@ -434,27 +444,31 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     switch (L) { BCMGX_SY(4) BCMGX_SY(8) BCMGX_SY(16) BCMGX_SY(32) }
 #undef BCMGX_SY
     BCMGX_LAUNCHED();
-    DBuf<int> rank(c, mm), inv(c, mm), tkey(c, mm);
-    DBuf<int> S(c, n);
-    fill<int>(c, S.get(), n, 0);
+    DBuf<int> rank(c, mm);
+    DBuf<int2> pref(c, mm);
+    DBuf<long long> S(c, n);
+    fill<long long>(c, S.get(), n, 0);
 #define BCMGX_MT(LL)                                                                                                   \
     case LL:                                                                                                           \
-        k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(), inv.get()); \
+        k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get());    \
         BCMGX_LAUNCHED();                                                                                              \
-        k_tkey<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), tkey.get());     \
+        k_pref<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(),     \
+                                                               pref.get());                                            \
         BCMGX_LAUNCHED();                                                                                              \
         break;
     switch (L) { BCMGX_MT(4) BCMGX_MT(8) BCMGX_MT(16) BCMGX_MT(32) }
 #undef BCMGX_MT
-    // proposals are latency-bound: half as many lanes per vertex (two entries
-    // each) keeps twice as many proposal chains in flight per wave
-    int LS = L > 4 ? L / 2 : L;
+    auto* Su = reinterpret_cast<unsigned long long*>(S.get());
+    // proposals are latency-bound: the first SUITOR_LANES candidates of the
+    // preference list usually hold the taken one, and fewer lanes per vertex
+    // keep more proposal chains in flight per wave
+    int LS = std::min(L, SUITOR_LANES);
     switch (LS) {
-    case 4: suitor_rounds<4>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get()); break;
-    case 8: suitor_rounds<8>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get()); break;
-    default: suitor_rounds<16>(c, n, Wrow.get(), Wcol.get(), Wval.get(), tkey.get(), inv.get(), S.get()); break;
+    case 4: suitor_rounds<4>(c, n, Wrow.get(), pref.get(), Su); break;
+    case 8: suitor_rounds<8>(c, n, Wrow.get(), pref.get(), Su); break;
+    default: suitor_rounds<16>(c, n, Wrow.get(), pref.get(), Su); break;
     }
-    k_suitor_decode<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Wrow.get(), inv.get(), S.get(), M);
+    k_suitor_decode<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Su, M);
     BCMGX_LAUNCHED();
 }
 

@@ -114,6 +114,21 @@ def test_matching_bit_exact(ctx, name):
     assert np.array_equal(A.matching(w), O.suitor(oa, w))
 
 
+@pytest.mark.parametrize("name", ["lap27_40", "lap7_64", "long_rows", "long_rows_ties"])
+def test_matching_chains_and_long_rows(ctx, name):
+    """Grids large enough for displacement chains to span several rounds, and
+    rows longer than the proposal group (the preference list is scanned in
+    several slices)."""
+    oa = {"lap27_40": lambda: O.laplacian3d(27, 40, 40, 40),
+          "lap7_64": lambda: O.laplacian3d(7, 64, 64, 64),
+          "long_rows": SPMM_CASES["long_rows"],
+          "long_rows_ties": lambda: O.Csr.from_scipy(rand_spd(400, 0.3, 9, ties=True))}[name]()
+    A = dev(ctx, oa)
+    for seed in (None, 3):
+        w = np.ones(A.n) if seed is None else np.random.default_rng(seed).uniform(0.5, 2.0, A.n)
+        assert np.array_equal(A.matching(w), O.suitor(oa, w)), seed
+
+
 def test_matching_bcsstk09(ctx, golden):
     oa = O.read_mtx(os.path.join(golden, "bcsstk09.mtx"))
     A = dev(ctx, oa)
