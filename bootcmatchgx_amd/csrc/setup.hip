@@ -196,32 +196,46 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
 template <int G>
 __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
                                              const double* __restrict__ Wval, int* __restrict__ rank) {
-    int y = (blockIdx.x * WG + (int)threadIdx.x) / G;
-    int lane = (int)threadIdx.x % G;
+    // a row of at most G entries: one entry per lane, staged in LDS, where the
+    // group reads every entry of the row by broadcast
+    __shared__ double sw[WG];
+    __shared__ int sc[WG];
+    int tid = (int)threadIdx.x;
+    int y = (blockIdx.x * WG + tid) / G;
+    int lane = tid % G;
+    int gb = tid - lane;
+    int s = 0, e = 0;
+    if (y < n) {
+        s = Wrow[y];
+        e = Wrow[y + 1];
+    }
+    int len = e - s;
+    bool va = y < n && len <= G && lane < len;
+    double wa = va ? Wval[s + lane] : 0.0;
+    int ca = va ? Wcol[s + lane] : 0;
+    sw[tid] = wa;
+    sc[tid] = ca;
+    __syncthreads();
     if (y >= n) return;
-    int s = Wrow[y], e = Wrow[y + 1];
-    if (e - s <= G) {  // uniform over the group: one entry per lane, compared through shuffles
-        int len = e - s;
-        bool va = lane < len;
-        double wa = va ? Wval[s + lane] : 0.0;
-        int ca = va ? Wcol[s + lane] : 0;
-        int r = 0;
-#pragma unroll
-        for (int b = 0; b < G; b++) {
-            double wb = __shfl(wa, b, G);
-            int cb = __shfl(ca, b, G);
-            r += (b < len && (wb < wa || (wb == wa && cb < ca))) ? 1 : 0;
+    if (len <= G) {  // uniform over the group
+        if (va) {
+            int r = 0;
+            for (int b = 0; b < len; b++) {
+                double wb = sw[gb + b];
+                int cb = sc[gb + b];
+                r += (wb < wa || (wb == wa && cb < ca)) ? 1 : 0;
+            }
+            rank[s + lane] = r;
         }
-        if (va) rank[s + lane] = r;
         return;
     }
     for (int a = s + lane; a < e; a += G) {
-        double wa = Wval[a];
-        int ca = Wcol[a];
+        double w = Wval[a];
+        int cw = Wcol[a];
         int r = 0;
         for (int b = s; b < e; b++) {
             double wb = Wval[b];
-            r += (wb < wa || (wb == wa && Wcol[b] < ca)) ? 1 : 0;
+            r += (wb < w || (wb == w && Wcol[b] < cw)) ? 1 : 0;
         }
         rank[a] = r;
     }
@@ -253,7 +267,7 @@ __global__ __launch_bounds__(WG) void k_pref(int n, const int* __restrict__ Wrow
 }
 
 // lanes per proposing vertex (the candidates one ballot looks at)
-constexpr int SUITOR_LANES = 16;
+constexpr int SUITOR_LANES = 8;
 
 // Lock-free Suitor.  S[y] = (1 + rank of y's current suitor) << 32 | suitor
 // (0: none).  A vertex proposes to its best neighbour -- positive weight,
