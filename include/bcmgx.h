@@ -89,7 +89,8 @@ typedef struct bcmgx_params {
     int autotune;           /* "autotune": pick SpMV/SpMM lanes-per-row by timing (default 1) */
     int check_every;        /* "check_every": FCG iterations between host convergence polls (default 8) */
     int fcg_z32;            /* "fcg_z32": with an FP32 hierarchy, FCG keeps the cycle's result z in FP32 and
-                               widens it inside its kernels (same results, fewer bytes; default 1) */
+                               widens it exactly inside its kernels (fewer bytes; the dot products are blocked
+                               differently, so iterates may differ in the last bits; default 1) */
 } bcmgx_params;
 
 /* Defaults of src/config/Params.h:238-291. */
