@@ -267,7 +267,7 @@ __global__ __launch_bounds__(WG) void k_pref(int n, const int* __restrict__ Wrow
 }
 
 // lanes per proposing vertex (the candidates one ballot looks at)
-constexpr int SUITOR_LANES = 8;
+constexpr int SUITOR_LANES = 4;
 
 // Lock-free Suitor.  S[y] = (1 + rank of y's current suitor) << 32 | suitor
 // (0: none).  A vertex proposes to its best neighbour -- positive weight,
@@ -284,14 +284,20 @@ constexpr int SUITOR_LANES = 8;
 // most `maxsteps` proposals; a vertex still unsettled then goes to the next
 // round's list.  Bounding the steps keeps one long displacement chain from
 // serialising a whole launch: every round works on all pending vertices in
-// parallel with fresh offers.  A proposal step is a chain of dependent loads
-// (row bounds, preference list, suitor words, atomicMax), so it is latency-
-// bound: the preference list in order lets the group take the first eligible
-// neighbour of its first G (one ballot, no weight comparisons), and the suitor
-// word names the displaced vertex without a lookup.
+// parallel with fresh offers.
+//
+// The rounds are bound by the throughput of the device-scope suitor-word
+// reads, so each vertex keeps a cursor into its preference list: pos[v] is the
+// list position of v's last offer.  A neighbour that would not accept v (its
+// suitor word beats v's key there) never will, since suitor words only rise;
+// so v resumes after its cursor and reads only the suitor words of neighbours
+// it has not yet been refused by.  The group reads G list entries and G suitor
+// words at a time and takes the first that would accept (one ballot).  A stale
+// read of a cursor (another XCD's write not yet visible) only lowers it, which
+// costs reads, never a candidate.
 template <int G>
 __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wrow, const int2* __restrict__ pref,
-                                               unsigned long long* S, const int* __restrict__ in_list,
+                                               int* pos, unsigned long long* S, const int* __restrict__ in_list,
                                                const int* __restrict__ in_count, int* __restrict__ out_list,
                                                int* __restrict__ out_count, int maxsteps) {
     static_assert(G <= 32, "group mask");
@@ -303,18 +309,22 @@ __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wr
         // round 0 visits the vertices from the highest id down: a suitee prefers
         // the higher suitor, so offers made in this order are rarely displaced
         int cur = in_list ? in_list[it] : n - 1 - it;
+        int p = in_list ? pos[cur] + 1 : Wrow[cur];  // next list position to try
+        int e = Wrow[cur + 1];
         int steps = 0;
         while (cur >= 0) {  // uniform inside the group
             if (steps == maxsteps) {
-                if (lane == 0) out_list[atomicAdd(out_count, 1)] = cur;
+                if (lane == 0) {
+                    pos[cur] = p - 1;
+                    out_list[atomicAdd(out_count, 1)] = cur;
+                }
                 break;
             }
             steps++;
-            int s = Wrow[cur], e = Wrow[cur + 1];
-            int best = -1, bkey = 0;
-            for (int b = s; b < e; b += G) {  // G candidates at a time, best first
+            int best = -1, bkey = 0, bidx = -1;
+            for (; p < e; p += G) {  // G candidates at a time, best first
                 int2 pk = make_int2(-1, 0);
-                if (b + lane < e) pk = pref[b + lane];
+                if (p + lane < e) pk = pref[p + lane];
                 bool ok = false;
                 // (the range test only matters for rows with repeated columns,
                 // which leave slots of the list unwritten)
@@ -327,12 +337,14 @@ __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wr
                     int f = __ffsll(m) - 1;
                     best = __shfl(pk.x, f, G);
                     bkey = __shfl(pk.y, f, G);
+                    bidx = p + f;
                     break;
                 }
             }
-            if (best < 0) break;
+            if (best < 0) break;  // no neighbour left that would accept cur
             unsigned olo = 0, ohi = 0;
             if (lane == 0) {
+                pos[cur] = bidx;
                 unsigned long long offer = ((unsigned long long)(unsigned)bkey << 32) | (unsigned)cur;
                 unsigned long long old = atomicMax(&S[best], offer);
                 olo = (unsigned)old;
@@ -340,9 +352,23 @@ __global__ __launch_bounds__(WG) void k_suitor(int n, const int* __restrict__ Wr
             }
             olo = __shfl(olo, 0, G);
             ohi = __shfl(ohi, 0, G);
-            if ((int)ohi < bkey) cur = ohi > 0 ? (int)olo : -1;
+            if ((int)ohi < bkey) {  // installed; the displaced suitor (if any) resumes after its cursor
+                cur = ohi > 0 ? (int)olo : -1;
+                if (cur >= 0) {
+                    p = pos[cur] + 1;
+                    e = Wrow[cur + 1];
+                }
+            } else {
+                p = bidx + 1;  // refused: a better offer holds that neighbour
+            }
         }
     }
+}
+
+// cursors before round 0: every list is untried
+__global__ void k_pos_init(int n, const int* __restrict__ Wrow, int* __restrict__ pos) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) pos[v] = Wrow[v] - 1;
 }
 
 __global__ void k_suitor_decode(int n, const unsigned long long* __restrict__ S, int* __restrict__ M) {
@@ -366,7 +392,7 @@ __global__ void k_make_wrow(int n, const int* __restrict__ row, int* __restrict_
 // Rounds of k_suitor until no vertex is pending.  The list length is read
 // back every few rounds (an empty round costs one tiny launch).
 template <int G>
-static void suitor_rounds(Context* c, int n, const int* Wrow, const int2* pref, unsigned long long* S) {
+static void suitor_rounds(Context* c, int n, const int* Wrow, const int2* pref, int* pos, unsigned long long* S) {
     // proposals per vertex per launch: short while most vertices are pending
     // (the launch is throughput-bound), longer once only chains remain (each
     // launch is then latency-bound and fewer launches finish them sooner)
@@ -377,7 +403,7 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int2* pref, 
     DBuf<int> la(c, n), lb(c, n), cnt(c, RING);
     fill<int>(c, cnt.get(), RING, 0);
     unsigned grid = (unsigned)std::min<long long>(grid_for(n, WG / G), (long long)c->num_cus * 16);
-    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, pref, S, nullptr, nullptr, la.get(), cnt.get(),
+    k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, pref, pos, S, nullptr, nullptr, la.get(), cnt.get(),
                                              maxsteps_of(0));
     BCMGX_LAUNCHED();
     int* lists[2] = {la.get(), lb.get()};
@@ -387,7 +413,7 @@ static void suitor_rounds(Context* c, int n, const int* Wrow, const int2* pref, 
         if (round % RING == 0) fill<int>(c, cnt.get(), RING - 1, 0);  // every counter but this round's input
         const int* in_cnt = cnt.get() + (round - 1) % RING;
         int* out_cnt = cnt.get() + round % RING;
-        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, pref, S, lists[cur], in_cnt, lists[cur ^ 1],
+        k_suitor<G><<<grid, WG, 0, c->stream>>>(n, Wrow, pref, pos, S, lists[cur], in_cnt, lists[cur ^ 1],
                                                  out_cnt, maxsteps_of(round));
         BCMGX_LAUNCHED();
         cur ^= 1;
@@ -460,8 +486,13 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     BCMGX_LAUNCHED();
     DBuf<int> rank(c, mm);
     DBuf<int2> pref(c, mm);
+    DBuf<int> pos(c, n);
     DBuf<long long> S(c, n);
     fill<long long>(c, S.get(), n, 0);
+    // the cursors start below every list: a read that misses a later write
+    // (not yet visible across XCDs) still sees a valid lower bound
+    k_pos_init<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Wrow.get(), pos.get());
+    BCMGX_LAUNCHED();
 #define BCMGX_MT(LL)                                                                                                   \
     case LL:                                                                                                           \
         k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get());    \
@@ -478,9 +509,10 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     // keep more proposal chains in flight per wave
     int LS = std::min(L, SUITOR_LANES);
     switch (LS) {
-    case 4: suitor_rounds<4>(c, n, Wrow.get(), pref.get(), Su); break;
-    case 8: suitor_rounds<8>(c, n, Wrow.get(), pref.get(), Su); break;
-    default: suitor_rounds<16>(c, n, Wrow.get(), pref.get(), Su); break;
+    case 2: suitor_rounds<2>(c, n, Wrow.get(), pref.get(), pos.get(), Su); break;
+    case 4: suitor_rounds<4>(c, n, Wrow.get(), pref.get(), pos.get(), Su); break;
+    case 8: suitor_rounds<8>(c, n, Wrow.get(), pref.get(), pos.get(), Su); break;
+    default: suitor_rounds<16>(c, n, Wrow.get(), pref.get(), pos.get(), Su); break;
     }
     k_suitor_decode<<<grid_for(n, WG), WG, 0, c->stream>>>(n, Su, M);
     BCMGX_LAUNCHED();
