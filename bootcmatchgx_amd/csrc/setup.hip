@@ -173,10 +173,12 @@ __device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
     return lo;
 }
 
-// _write_T_warp (matching.cu:29-64): W(i, c) <- W(c, i) for c < i
+// _write_T_warp (matching.cu:29-64): W(i, c) <- W(c, i) for c < i.  The
+// mirrored pair (i, c) / (c, i) found here is recorded both ways in mir
+// (filled with -1 beforehand: entries without a mirror keep it).
 template <int L>
 __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                                   double* __restrict__ Wval) {
+                                                   double* __restrict__ Wval, int* __restrict__ mir) {
     int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
     int lane = (int)threadIdx.x % L;
     if (i >= n) return;
@@ -185,7 +187,11 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
         if (cc >= i) break;
         int b = Wrow[cc], len = Wrow[cc + 1] - b;
         int p = lower_bound_dev(Wcol + b, len, i);
-        if (p < len && Wcol[b + p] == i) Wval[j] = Wval[b + p];  // mirrored entry present
+        if (p < len && Wcol[b + p] == i) {  // mirrored entry present
+            Wval[j] = Wval[b + p];
+            mir[j] = b + p;
+            mir[b + p] = j;
+        }
     }
 }
 
@@ -249,19 +255,15 @@ __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow
 template <int G>
 __global__ __launch_bounds__(WG) void k_pref(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
                                              const double* __restrict__ Wval, const int* __restrict__ rank,
-                                             int2* __restrict__ pref) {
+                                             const int* __restrict__ mir, int2* __restrict__ pref) {
     int u = (blockIdx.x * WG + (int)threadIdx.x) / G;
     int lane = (int)threadIdx.x % G;
     if (u >= n) return;
     int s = Wrow[u], e = Wrow[u + 1];
     for (int a = s + lane; a < e; a += G) {
         int y = Wcol[a];
-        int key = 0;
-        if (Wval[a] > 0 && y >= 0 && y < n) {
-            int b = Wrow[y], len = Wrow[y + 1] - b;
-            int p = lower_bound_dev(Wcol + b, len, u);
-            if (p < len && Wcol[b + p] == u) key = rank[b + p] + 1;
-        }
+        int q = mir[a];  // u's entry in y's row, -1 when absent
+        int key = (Wval[a] > 0 && y >= 0 && y < n && q >= 0) ? rank[q] + 1 : 0;
         pref[e - 1 - rank[a]] = make_int2(y, key);
     }
 }
@@ -477,9 +479,11 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
         k_make_W<<<grid_for(m, WG), WG, 0, c->stream>>>(m, Wval.get(), mn.get());
         BCMGX_LAUNCHED();
     }
-#define BCMGX_SY(LL)                                                                                         \
-    case LL:                                                                                                 \
-        k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get()); \
+    DBuf<int> mir(c, mm);
+    fill<int>(c, mir.get(), (long long)mm, -1);
+#define BCMGX_SY(LL)                                                                                                     \
+    case LL:                                                                                                             \
+        k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), mir.get()); \
         break;
     switch (L) { BCMGX_SY(4) BCMGX_SY(8) BCMGX_SY(16) BCMGX_SY(32) }
 #undef BCMGX_SY
@@ -498,7 +502,7 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
         k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get());    \
         BCMGX_LAUNCHED();                                                                                              \
         k_pref<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(),     \
-                                                               pref.get());                                            \
+                                                               mir.get(), pref.get());                                 \
         BCMGX_LAUNCHED();                                                                                              \
         break;
     switch (L) { BCMGX_MT(4) BCMGX_MT(8) BCMGX_MT(16) BCMGX_MT(32) }
