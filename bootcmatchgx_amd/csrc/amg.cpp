@@ -655,7 +655,7 @@ static SolveWork& solve_work(Context* c, const DCsr& A, Prec* prec, const bcmgx_
         s.z32_blocks = spmm_dots_z32_blocks(c, n, k);
         s.partials.reset(c, (size_t)std::max({s.fb.nblk, s.mv_blocks, s.z32_blocks}) * k * 3);
         s.st.reset(c, k);
-        s.flags.reset(c, 2);
+        s.flags.reset(c, 3);
         s.fb.partials = s.partials.get();
         s.fb.st = s.st.get();
         s.fb.all_done = s.flags.get();
@@ -665,7 +665,8 @@ static SolveWork& solve_work(Context* c, const DCsr& A, Prec* prec, const bcmgx_
     return *sp;
 }
 
-SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k) {
+SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k,
+                      const SolveIo* io) {
     BCMGX_CHECK(k >= 1 && k <= 64, BCMGX_ERR_ARG, "nrhs must be in [1, 64]");
     BCMGX_CHECK(p.solver == BCMGX_SOLVER_FCG, BCMGX_ERR_STATE, "fcg_solve called for another solver");
     double t0 = now_s();
@@ -682,15 +683,25 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     s.z32 = z32;
     if (s.z32 && !s.zf) s.zf.reset(c, (size_t)std::max(n, 1) * k);
     long long nk = (long long)n * k;
-    copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
-    copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
+    int* xzero = s.flags.get() + 2;  // the initial guess holds only zeros (set while staging it)
+    if (io) {
+        fill<int>(c, xzero, 1, 1);
+        colmajor_to_interleaved(c, io->b, io->ld, s.B.get(), n, k);
+        colmajor_to_interleaved(c, io->x, io->ld, s.X.get(), n, k, xzero);
+    } else {
+        copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
+        copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
+    }
     fill<int>(c, s.flags.get(), 2, 0);
     // ---- initial step (FCG.cu:44-99)
     {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
         a.x = s.X.get(); a.y = s.r.get(); a.b = s.B.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
+        a.stop = io ? xzero : nullptr;
         spmv<double, double>(c, SpOp::RES, a);  // r = b - A x
+        // a zero guess: every row sum of A x is +0, so b - A x is b itself
+        if (io) copy_if_flag(c, xzero, s.B.get(), s.r.get(), nk);
     }
     fcg_norm(c, s.fb, s.r.get(), 0);           // delta0 = ||r||
     prec->apply(s.r.get(), s.v.get(), k, true, nullptr);
@@ -737,8 +748,14 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     }
     // every column stopped after the first iteration of the last pair: its x step waits in pend
     // and its direction is still in v (the first iteration's p0), or in d with z in FP32
-    fcg_flush_pending(c, s.fb, s.z32 ? s.d.get() : s.v.get(), s.X.get());
-    copy_d2d(c, x, s.X.get(), sizeof(double) * nk);
+    const double* pw = s.z32 ? s.d.get() : s.v.get();
+    if (io) {  // the flush rides on the way out
+        interleaved_to_colmajor(c, s.X.get(), io->x, io->ld, n, k, s.fb.st, pw);
+        fcg_clear_pending(c, s.fb);
+    } else {
+        fcg_flush_pending(c, s.fb, pw, s.X.get());
+        copy_d2d(c, x, s.X.get(), sizeof(double) * nk);
+    }
     std::vector<FcgState> st(k);
     c->d2h(st.data(), s.st.get(), k);
     SolveResult res;

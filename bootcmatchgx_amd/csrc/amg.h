@@ -89,7 +89,16 @@ struct SolveResult {
     double exit_res = 0, rel_res = 0, del0 = 0, seconds = 0;
 };
 // FCG on k interleaved columns (device pointers b, x; x in/out)
-SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
+// the caller's column-major device blocks (leading dimension ld), staged
+// straight into and out of the solver's interleaved buffers
+struct SolveIo {
+    const double* b = nullptr;
+    double* x = nullptr;
+    int ld = 0;
+};
+// with io, b and x are ignored and the blocks come from io
+SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k,
+                      const SolveIo* io = nullptr);
 // CG (Hestenes-Stiefel) on k interleaved columns, same conventions
 SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params& p, const double* b, double* x, int k);
 // s-step CG (Chronopoulos-Gear; p.solver CGS or CGS_CUBLAS) or its pipelined

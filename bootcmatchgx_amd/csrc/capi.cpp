@@ -419,14 +419,24 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
         own = prec_setup(c, A->A, q);
         P = own.get();
     }
-    size_t nk = (size_t)std::max(n, 1) * nrhs;
-    DBuf<double> B(c, nk), X(c, nk), tmp;
-    stage_in(c, b, ld, mem, n, nrhs, B.get(), tmp);
-    stage_in(c, x, ld, mem, n, nrhs, X.get(), tmp);
-    SolveResult r = p->solver == BCMGX_SOLVER_CGHS  ? cghs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
-                    : p->solver != BCMGX_SOLVER_FCG ? cgs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
-                                                    : fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
-    stage_out(c, X.get(), x, ld, mem, n, nrhs, tmp);
+    SolveResult r;
+    if (p->solver == BCMGX_SOLVER_FCG && mem != BCMGX_MEM_HOST && nrhs > 1 && n > 0) {
+        // device blocks: FCG stages them into its own buffers
+        SolveIo io;
+        io.b = b;
+        io.x = x;
+        io.ld = ld;
+        r = fcg_solve(c, A->A, P, *p, nullptr, nullptr, nrhs, &io);
+    } else {
+        size_t nk = (size_t)std::max(n, 1) * nrhs;
+        DBuf<double> B(c, nk), X(c, nk), tmp;
+        stage_in(c, b, ld, mem, n, nrhs, B.get(), tmp);
+        stage_in(c, x, ld, mem, n, nrhs, X.get(), tmp);
+        r = p->solver == BCMGX_SOLVER_CGHS  ? cghs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
+            : p->solver != BCMGX_SOLVER_FCG ? cgs_solve(c, A->A, P, *p, B.get(), X.get(), nrhs)
+                                            : fcg_solve(c, A->A, P, *p, B.get(), X.get(), nrhs);
+        stage_out(c, X.get(), x, ld, mem, n, nrhs, tmp);
+    }
     c->sync();
     if (out) {
         out->retv = r.retv;

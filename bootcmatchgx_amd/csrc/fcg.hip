@@ -394,6 +394,11 @@ __global__ void k_clear_pending(int k, FcgState* st) {
     if (c < k) st[c].pad1 = 0;
 }
 
+void fcg_clear_pending(Context* c, const FcgBufs& f) {
+    k_clear_pending<<<1, 64, 0, c->stream>>>(f.k, f.st);
+    BCMGX_LAUNCHED();
+}
+
 void fcg_flush_pending(Context* c, const FcgBufs& f, const double* t, double* x) {
     int KP = kpad(f.k);
     k_flush_pending<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, f.st, t, x);

@@ -19,8 +19,6 @@ void fill(Context* c, T* p, long long n, T v);
 template <typename A, typename B>
 void convert(Context* c, const A* s, B* d, long long n);
 void copy_d2d(Context* c, void* dst, const void* src, size_t bytes);
-void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k);
-void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k);
 double stream_copy_gbs(Context* c, size_t bytes, int reps);
 
 // ---- sparse matrix-vector family (spmv.hip) ---------------------------------
@@ -91,6 +89,14 @@ struct FcgState {
     double pend;  // FCG: theta2 of the first iteration of a pair, applied to x by the second
     int iter, done, pad0, pad1;  // FCG: pad1 = 1 while pend waits; CG: pad0 = converged
 };
+// column-major blocks (leading dimension ld) <-> k interleaved columns.
+// zero_flag (may be null) is cleared when the block holds a nonzero; st/p (may
+// be null) add the waiting FCG x step (pad1, pend) of each column on the way out.
+void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k, int* zero_flag = nullptr);
+void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k, const FcgState* st = nullptr,
+                             const double* p = nullptr);
+// d = s when *flag != 0 (n doubles)
+void copy_if_flag(Context* c, const int* flag, const double* s, double* d, long long n);
 struct FcgBufs {
     int n = 0, k = 1;
     int nblk = 0;              // partial-sum blocks (fixed for a given n)
@@ -134,6 +140,8 @@ void fcg_update_z32(Context* c, const FcgBufs& f, const float* z, double* p, con
                     int phase, const FcgEntry& e);
 // a step still waiting after the last pair (every column stopped after its
 // first iteration): x += pend t, t the first iteration's direction
+// clears every column's waiting x step (after it was applied elsewhere)
+void fcg_clear_pending(Context* c, const FcgBufs& f);
 void fcg_flush_pending(Context* c, const FcgBufs& f, const double* t, double* x);
 
 // ---- CG (Hestenes-Stiefel) kernels (fcg.hip), CG_HS.cu:24-172 ----------------

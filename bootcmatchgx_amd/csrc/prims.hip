@@ -253,29 +253,89 @@ void copy_d2d(Context* c, void* dst, const void* src, size_t bytes) {
     BCMGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
 }
 
-// column-major (ld) <-> interleaved row-major [n][k]
-template <typename T>
-__global__ void k_cm_to_rm(const T* __restrict__ s, int ld, T* __restrict__ d, int n, int k) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * k) return;
-    int i = (int)(t / k), col = (int)(t % k);
-    d[t] = s[(long long)col * ld + i];
+// column-major (ld) <-> interleaved row-major [n][k], through an LDS tile of
+// TR rows x k columns (row stride k + 1, so neither side's LDS accesses meet
+// the same bank), so that the column-major side and the interleaved side are
+// both read and written with consecutive lanes on consecutive addresses.
+constexpr int TILE_ELEMS = 2048;
+static int tile_rows(int k) { return std::max(1, TILE_ELEMS / k); }
+static size_t tile_bytes(int k) { return (size_t)tile_rows(k) * (k + 1) * sizeof(double); }
+
+// zero_flag (may be null) is cleared when any element read is nonzero
+__global__ __launch_bounds__(WG) void k_cm_to_rm(const double* __restrict__ s, int ld, double* __restrict__ d, int n,
+                                                 int k, int TR, int* __restrict__ zero_flag) {
+    extern __shared__ double tile[];
+    long long i0 = (long long)blockIdx.x * TR;
+    int rows = (int)std::min<long long>(TR, n - i0);
+    bool nz = false;
+    for (int idx = threadIdx.x; idx < TR * k; idx += WG) {
+        int r = idx % TR, c = idx / TR;
+        if (r < rows) {
+            double v = s[(long long)c * ld + i0 + r];
+            nz |= v != 0.0;
+            tile[r * (k + 1) + c] = v;
+        }
+    }
+    if (zero_flag && nz) *zero_flag = 0;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < rows * k; idx += WG) {
+        int r = idx / k, c = idx % k;
+        d[(i0 + r) * k + c] = tile[r * (k + 1) + c];
+    }
 }
-template <typename T>
-__global__ void k_rm_to_cm(const T* __restrict__ s, T* __restrict__ d, int ld, int n, int k) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * k) return;
-    int i = (int)(t / k), col = (int)(t % k);
-    d[(long long)col * ld + i] = s[t];
+
+// st/p (may be null): columns whose FCG step waits (pad1) get x + pend * p on the way out
+__global__ __launch_bounds__(WG) void k_rm_to_cm(const double* __restrict__ s, double* __restrict__ d, int ld, int n,
+                                                 int k, int TR, const FcgState* __restrict__ st,
+                                                 const double* __restrict__ p) {
+    extern __shared__ double tile[];
+    __shared__ double pend[64];
+    __shared__ int waits[64];
+    if (threadIdx.x < k) {
+        waits[threadIdx.x] = st ? st[threadIdx.x].pad1 : 0;
+        pend[threadIdx.x] = st ? st[threadIdx.x].pend : 0.0;
+    }
+    __syncthreads();
+    long long i0 = (long long)blockIdx.x * TR;
+    int rows = (int)std::min<long long>(TR, n - i0);
+    for (int idx = threadIdx.x; idx < rows * k; idx += WG) {
+        int r = idx / k, c = idx % k;
+        long long o = (i0 + r) * k + c;
+        double v = s[o];
+        if (waits[c]) v = pend[c] * p[o] + v;  // the flush of k_flush_pending
+        tile[r * (k + 1) + c] = v;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < TR * k; idx += WG) {
+        int r = idx % TR, c = idx / TR;
+        if (r < rows) d[(long long)c * ld + i0 + r] = tile[r * (k + 1) + c];
+    }
 }
-void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k) {
+
+void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k, int* zero_flag) {
     if ((long long)n * k == 0) return;
-    k_cm_to_rm<double><<<grid_for((long long)n * k, WG), WG, 0, c->stream>>>(s, ld, d, n, k);
+    int TR = tile_rows(k);
+    k_cm_to_rm<<<(unsigned)((n + TR - 1) / TR), WG, tile_bytes(k), c->stream>>>(s, ld, d, n, k, TR, zero_flag);
     BCMGX_LAUNCHED();
 }
-void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k) {
+void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k, const FcgState* st,
+                             const double* p) {
     if ((long long)n * k == 0) return;
-    k_rm_to_cm<double><<<grid_for((long long)n * k, WG), WG, 0, c->stream>>>(s, d, ld, n, k);
+    BCMGX_CHECK(k <= 64, BCMGX_ERR_ARG, "at most 64 columns");
+    int TR = tile_rows(k);
+    k_rm_to_cm<<<(unsigned)((n + TR - 1) / TR), WG, tile_bytes(k), c->stream>>>(s, d, ld, n, k, TR, st, p);
+    BCMGX_LAUNCHED();
+}
+
+__global__ void k_copy_if(const int* __restrict__ flag, const double* __restrict__ s, double* __restrict__ d, long long n) {
+    if (!*flag) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        d[t] = s[t];
+}
+void copy_if_flag(Context* c, const int* flag, const double* s, double* d, long long n) {
+    if (n <= 0) return;
+    unsigned g = (unsigned)std::min<long long>(grid_for(n, WG), (long long)c->num_cus * 8);
+    k_copy_if<<<g, WG, 0, c->stream>>>(flag, s, d, n);
     BCMGX_LAUNCHED();
 }
 

@@ -473,6 +473,39 @@ def test_missing_diagonal_is_reported(ctx):
     assert "row 7 has no diagonal" in str(e.value)
 
 
+@pytest.mark.parametrize("prec,k,guess", [("FP32", 32, "zero"), ("FP32", 32, "rand"), ("FP64", 5, "zero"),
+                                           ("FP64", 2, "rand"), ("FP32", 12, "neg0"), ("FP64", 64, "zero")])
+def test_fcg_device_blocks_match_host_path(ctx, prec, k, guess):
+    """FCG on column-major device blocks stages them straight into its buffers
+    (tiled layout change, the residual of a zero guess taken as b, the waiting x
+    step added on the way out); it gives the host path's bits, leaves padding
+    rows alone and counts the same iterations."""
+    oa = O.laplacian3d(27, 14, 13, 12)
+    A = dev(ctx, oa)
+    n = A.n
+    ld = n + 3
+    rng = np.random.default_rng(k)
+    B = rng.normal(size=(n, k))
+    X0 = {"zero": np.zeros((n, k)), "rand": rng.normal(size=(n, k)) * 0.1, "neg0": np.full((n, k), -0.0)}[guess]
+    p = params("BCMG", precision=prec, rtol=1e-8)
+    P = bx.prec_setup(ctx, A, p)
+    xh, oh = bx.solve(ctx, A, B, X0, p, P)
+    pad = 7.25
+    Bd, Xd = bx.DeviceArray(ctx, (ld, k)), bx.DeviceArray(ctx, (ld, k))
+    bb = np.full((ld, k), pad)
+    bb[:n] = B
+    xx = np.full((ld, k), pad)
+    xx[:n] = X0
+    Bd.upload(bb.T.ravel())
+    Xd.upload(xx.T.ravel())
+    od = bx.solve_device(ctx, A, Bd.ptr, Xd.ptr, k, p, P, ld=ld)
+    xd = Xd.download().reshape(k, ld).T
+    assert od["niter"] == oh["niter"]
+    assert np.array_equal(xd[:n], np.asarray(xh).reshape(n, k))
+    assert np.all(xd[n:] == pad)
+    P.finalize()
+
+
 @pytest.mark.parametrize("k", [4, 8, 32, 12, 64])
 def test_fcg_fp32_z_matches_widened_path(ctx, k):
     """With an FP32 hierarchy FCG keeps the cycle's result in FP32 (fcg_z32) and widens it inside
