@@ -686,8 +686,10 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     int* xzero = s.flags.get() + 2;  // the initial guess holds only zeros (set while staging it)
     if (io) {
         fill<int>(c, xzero, 1, 1);
-        colmajor_to_interleaved(c, io->b, io->ld, s.B.get(), n, k);
         colmajor_to_interleaved(c, io->x, io->ld, s.X.get(), n, k, xzero);
+        // a zero guess: every row sum of A x is +0, so r = b - A x is b itself
+        // and b goes straight to r
+        colmajor_to_interleaved(c, io->b, io->ld, s.B.get(), n, k, nullptr, xzero, s.r.get());
     } else {
         copy_d2d(c, s.B.get(), b, sizeof(double) * nk);
         copy_d2d(c, s.X.get(), x, sizeof(double) * nk);
@@ -700,11 +702,14 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         a.x = s.X.get(); a.y = s.r.get(); a.b = s.B.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         a.stop = io ? xzero : nullptr;
         spmv<double, double>(c, SpOp::RES, a);  // r = b - A x
-        // a zero guess: every row sum of A x is +0, so b - A x is b itself
-        if (io) copy_if_flag(c, xzero, s.B.get(), s.r.get(), nk);
     }
-    fcg_norm(c, s.fb, s.r.get(), 0);           // delta0 = ||r||
-    prec->apply(s.r.get(), s.v.get(), k, true, nullptr);
+    if (s.fused) {  // delta0 = ||r||, and the FP32 cycle's entry from r in the same pass
+        fcg_norm_entry(c, s.fb, s.r.get(), 0, s.entry);
+        prec->apply(s.r.get(), s.v.get(), k, true, nullptr, true);
+    } else {
+        fcg_norm(c, s.fb, s.r.get(), 0);  // delta0 = ||r||
+        prec->apply(s.r.get(), s.v.get(), k, true, nullptr);
+    }
     {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;

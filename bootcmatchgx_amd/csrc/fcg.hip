@@ -117,6 +117,42 @@ __global__ __launch_bounds__(WG) void k_norm_final(const double* __restrict__ pa
     }
 }
 
+// ||r||^2 partials in k_dots<1>'s blocks and order, with the FP32 cycle's
+// level-0 entry (F = (float) r, X = F/M) written on the way, as k_update does
+__global__ __launch_bounds__(WG) void k_norm_entry(int n, int k, int KP, const double* __restrict__ r,
+                                                   double* __restrict__ partials, FcgEntry e) {
+    __shared__ double s[WG];
+    int tid = threadIdx.x;
+    int R = WG / KP;
+    int rl = tid / KP, c = tid % KP;
+    double acc = 0.0;
+    if (c < k) {
+        for (long long i = (long long)blockIdx.x * R + rl; i < n; i += (long long)gridDim.x * R) {
+            long long o = i * k + c;
+            double rr = r[o];
+            acc += rr * rr;
+            float fv = (float)rr;
+            e.F[o] = fv;
+            e.X[o] = fv / e.M[i];
+        }
+    }
+    s[tid] = acc;
+    __syncthreads();
+    for (int st = WG / 2; st >= KP; st >>= 1) {
+        if (tid < st) s[tid] += s[tid + st];
+        __syncthreads();
+    }
+    if (tid < k) partials[((long long)blockIdx.x * k + tid) * 3] = s[tid];
+}
+
+void fcg_norm_entry(Context* c, const FcgBufs& f, const double* r, int mode, const FcgEntry& e) {
+    int KP = kpad(f.k);
+    k_norm_entry<<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, f.partials, e);
+    BCMGX_LAUNCHED();
+    k_norm_final<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st, mode, f.all_done, f.all_done + 1);
+    BCMGX_LAUNCHED();
+}
+
 void fcg_norm(Context* c, const FcgBufs& f, const double* v, int mode) {
     int KP = kpad(f.k);
     k_dots<1><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, v, v, nullptr, nullptr, nullptr, nullptr, f.partials, nullptr);

@@ -90,13 +90,13 @@ struct FcgState {
     int iter, done, pad0, pad1;  // FCG: pad1 = 1 while pend waits; CG: pad0 = converged
 };
 // column-major blocks (leading dimension ld) <-> k interleaved columns.
-// zero_flag (may be null) is cleared when the block holds a nonzero; st/p (may
-// be null) add the waiting FCG x step (pad1, pend) of each column on the way out.
-void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k, int* zero_flag = nullptr);
+// zero_flag (may be null) is cleared when the block holds a nonzero; the block
+// goes to alt instead of d when alt_if (may be null) is set and nonzero; st/p
+// (may be null) add the waiting FCG x step (pad1, pend) of each column on the way out.
+void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k, int* zero_flag = nullptr,
+                             const int* alt_if = nullptr, double* alt = nullptr);
 void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k, const FcgState* st = nullptr,
                              const double* p = nullptr);
-// d = s when *flag != 0 (n doubles)
-void copy_if_flag(Context* c, const int* flag, const double* s, double* d, long long n);
 struct FcgBufs {
     int n = 0, k = 1;
     int nblk = 0;              // partial-sum blocks (fixed for a given n)
@@ -117,6 +117,8 @@ struct FcgEntry {
     float* X = nullptr;
     const float* M = nullptr;
 };
+// fcg_norm of r that also writes the FP32 cycle's level-0 entry from r
+void fcg_norm_entry(Context* c, const FcgBufs& f, const double* r, int mode, const FcgEntry& e);
 // alpha = r.v, beta = w.v -> delta = beta, theta = alpha / delta  (FCG.cu:56-86)
 void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* v, const double* w);
 // x += theta v; r -= theta w  (FCG.cu:88-89); d = v, q = w

@@ -261,10 +261,13 @@ constexpr int TILE_ELEMS = 2048;
 static int tile_rows(int k) { return std::max(1, TILE_ELEMS / k); }
 static size_t tile_bytes(int k) { return (size_t)tile_rows(k) * (k + 1) * sizeof(double); }
 
-// zero_flag (may be null) is cleared when any element read is nonzero
+// zero_flag (may be null) is cleared when any element read is nonzero; the
+// block goes to alt instead of d when alt_if is set and nonzero
 __global__ __launch_bounds__(WG) void k_cm_to_rm(const double* __restrict__ s, int ld, double* __restrict__ d, int n,
-                                                 int k, int TR, int* __restrict__ zero_flag) {
+                                                 int k, int TR, int* __restrict__ zero_flag, const int* __restrict__ alt_if,
+                                                 double* __restrict__ alt) {
     extern __shared__ double tile[];
+    if (alt_if && *alt_if) d = alt;
     long long i0 = (long long)blockIdx.x * TR;
     int rows = (int)std::min<long long>(TR, n - i0);
     bool nz = false;
@@ -312,10 +315,12 @@ __global__ __launch_bounds__(WG) void k_rm_to_cm(const double* __restrict__ s, d
     }
 }
 
-void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k, int* zero_flag) {
+void colmajor_to_interleaved(Context* c, const double* s, int ld, double* d, int n, int k, int* zero_flag,
+                             const int* alt_if, double* alt) {
     if ((long long)n * k == 0) return;
     int TR = tile_rows(k);
-    k_cm_to_rm<<<(unsigned)((n + TR - 1) / TR), WG, tile_bytes(k), c->stream>>>(s, ld, d, n, k, TR, zero_flag);
+    k_cm_to_rm<<<(unsigned)((n + TR - 1) / TR), WG, tile_bytes(k), c->stream>>>(s, ld, d, n, k, TR, zero_flag, alt_if,
+                                                                                  alt);
     BCMGX_LAUNCHED();
 }
 void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int n, int k, const FcgState* st,
@@ -324,18 +329,6 @@ void interleaved_to_colmajor(Context* c, const double* s, double* d, int ld, int
     BCMGX_CHECK(k <= 64, BCMGX_ERR_ARG, "at most 64 columns");
     int TR = tile_rows(k);
     k_rm_to_cm<<<(unsigned)((n + TR - 1) / TR), WG, tile_bytes(k), c->stream>>>(s, d, ld, n, k, TR, st, p);
-    BCMGX_LAUNCHED();
-}
-
-__global__ void k_copy_if(const int* __restrict__ flag, const double* __restrict__ s, double* __restrict__ d, long long n) {
-    if (!*flag) return;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-        d[t] = s[t];
-}
-void copy_if_flag(Context* c, const int* flag, const double* s, double* d, long long n) {
-    if (n <= 0) return;
-    unsigned g = (unsigned)std::min<long long>(grid_for(n, WG), (long long)c->num_cus * 8);
-    k_copy_if<<<g, WG, 0, c->stream>>>(flag, s, d, n);
     BCMGX_LAUNCHED();
 }
 
