@@ -444,10 +444,17 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
         spmv<V, V>(c, SpOp::RES, a);
     }
     LevelWork<V>& wc = (*R.W)[l + 1];
+    // a coarse level below the coarsest starts with a sweep from zero, F / M:
+    // the restriction writes it along with F
+    bool sweep0 = l + 1 < nl - 1 && p.prerelax_sweeps >= 1;
     {
         SpmvArgs a;
         a.n = lv.R.n; a.row = lv.R.row.get(); a.col = lv.R.col.get(); a.val = mat_val<V>(lv.R); a.nnz = lv.R.nnz;
         a.x = w.Res.get(); a.y = wc.F.get(); a.k = k; a.lanes = lv.lanesR; a.stop = R.stop;
+        if (sweep0) {
+            a.x0 = wc.X.get();
+            a.D = lvl_M<V>(*H.L[l + 1]);
+        }
         spmv<V, V>(c, SpOp::AX, a);
     }
     V* xc = nullptr;
@@ -456,7 +463,7 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
         if (p.cycle_type == BCMGX_CYCLE_W && l < nl - 2) gs = 2;
         if (p.cycle_type == BCMGX_CYCLE_H && (l % 2) == 0) gs = 2;
         for (int i = 1; i <= gs; i++) {
-            xc = gamg_cycle<V>(R, l + 1, wc.F.get(), xc);
+            xc = gamg_cycle<V>(R, l + 1, wc.F.get(), xc, i == 1 && sweep0);
             if (l + 1 == nl - 1) break;
         }
     }

@@ -49,6 +49,7 @@ struct SpmvArgs {
                                 // super-groups of 2^s row groups)
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
     double* y64 = nullptr;      // JACOBI with V = float: write the result widened to FP64 here (y is scratch)
+    void* x0 = nullptr;         // AX: also write y / D here (V*; a first l1-Jacobi sweep from zero on y)
 };
 // T = matrix value type, V = vector type: (double,double), (float,float), (float,double)
 template <typename T, typename V>

@@ -182,6 +182,7 @@ __global__ __launch_bounds__(WG) void k_spmmv(int n, int k, const int* __restric
 // Operands of the pipelined SpMM's optional epilogues.
 struct SpmmExtra {
     double* y64 = nullptr;         // MODE 1: JACOBI result widened to FP64
+    void* x0 = nullptr;            // MODE 3 (AX): y / D, a first l1-Jacobi sweep from zero on y
     const double* r = nullptr;     // MODE 2: FCG residual
     const double* wold = nullptr;  // MODE 2: previous A-direction
     double* partials = nullptr;    // MODE 2: per-block dot partials
@@ -333,6 +334,9 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
             if constexpr (MODE == 1) {
                 VV res = ((-t + *(const VV*)(b + o)) / D[ri]) + __builtin_convertvector(*(const XV*)(x + o), VV);
                 *(VD*)(ex.y64 + o) = __builtin_convertvector(res, VD);
+            } else if constexpr (MODE == 3) {
+                *(VV*)(y + o) = t;
+                *(VV*)((V*)ex.x0 + o) = t / D[ri];
             } else if constexpr (MODE == 2) {
                 *(VV*)(y + o) = t;
                 VD pw = __builtin_convertvector(*(const XV*)(x + o), VD);
@@ -844,6 +848,27 @@ void spmv(Context* c, SpOp op, const SpmvArgs& a) {
             convert_stop<V, double>(c, (const V*)a.y, a.y64, (long long)a.n * a.k, a.stop);
             return;
         }
+    }
+    if (a.x0) {
+        // y = A x and x0 = y / D (a first sweep from zero on y): in one pass through
+        // the pipelined SpMM where it applies, else the product and then the sweep
+        if (op != SpOp::AX || !a.D || !a.y) throw Error(BCMGX_ERR_STATE, "spmv: x0 output only for y = A x with D");
+        constexpr int CPL = 16 / (int)sizeof(V);
+        bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)a.x0) & 15) == 0;
+        bool full = a.kchunk == 0 || a.kchunk >= a.k;
+        if (a.k > 1 && full && aligned && spmmp_lanes(a.k, CPL) && a.row && (a.nnz == 0 || (a.col && a.val && a.x))) {
+            SpmmExtra ex;
+            ex.x0 = a.x0;
+            SpmvArgs b = a;
+            b.kchunk = 0;
+            dispatch_spmmp<T, V, SpOp::AX, 3>(c, b, ex, 0);
+        } else {
+            SpmvArgs b = a;
+            b.x0 = nullptr;
+            spmv<T, V>(c, op, b);
+            jacobi_from_zero<V>(c, a.n, a.k, (const V*)a.y, (const V*)a.D, (V*)a.x0, a.stop);
+        }
+        return;
     }
     // operands the kernels dereference: a null one is a library bug, reported instead of launched
     if (!a.row || (a.nnz > 0 && (!a.col || !a.val || !a.x)) || !a.y || a.k < 1 ||
