@@ -111,56 +111,66 @@ __global__ void k_make_C(int n, const double* __restrict__ val, const int* __res
 
 // _makeAH_warp (matching.cu:66-111): off-diagonal AH_ij = 1 - 2 a_ij w_i w_j / (C_i + C_j),
 // stored on A's pattern without the diagonal (row i of W starts at row[i] - i).
+// The block also writes its min |AH| to bmin[blockIdx.x], for find_Max_Min
+// (op 1, matching.cu:188-217; a min is order independent).
 template <int L>
 __global__ __launch_bounds__(WG) void k_make_AH(int n, const int* __restrict__ row, const int* __restrict__ col,
                                                 const double* __restrict__ val, const int* __restrict__ dind,
                                                 const double* __restrict__ w, const double* __restrict__ C,
-                                                int* __restrict__ Wcol, double* __restrict__ Wval) {
+                                                int* __restrict__ Wcol, double* __restrict__ Wval,
+                                                double* __restrict__ bmin) {
+    __shared__ double sm[WG];
     int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
     int lane = (int)threadIdx.x % L;
-    if (i >= n) return;
-    int s = row[i], e = row[i + 1], d = dind[i];
-    for (int j = s + lane; j < e; j += L) {
-        if (j == d) continue;
-        int cc = col[j];
-        int o = j - i - (j > d ? 1 : 0);
-        int lo = cc < i ? cc : i, hi = cc < i ? i : cc;
-        double norm = C[lo] + C[hi];
-        double v;
-        if (norm > DBL_EPSILON) {
-            double wt = w[lo] * w[hi];
-            v = 1. - ((2. * val[j] * wt) / norm);
-        } else {
-            v = DBL_EPSILON;
+    double mn = DBL_MAX;
+    if (i < n) {
+        int s = row[i], e = row[i + 1], d = dind[i];
+        for (int j = s + lane; j < e; j += L) {
+            if (j == d) continue;
+            int cc = col[j];
+            int o = j - i - (j > d ? 1 : 0);
+            int lo = cc < i ? cc : i, hi = cc < i ? i : cc;
+            double norm = C[lo] + C[hi];
+            double v;
+            if (norm > DBL_EPSILON) {
+                double wt = w[lo] * w[hi];
+                v = 1. - ((2. * val[j] * wt) / norm);
+            } else {
+                v = DBL_EPSILON;
+            }
+            Wcol[o] = cc;
+            Wval[o] = v;
+            mn = fmin(mn, fabs(v));
         }
-        Wcol[o] = cc;
-        Wval[o] = v;
     }
+    sm[threadIdx.x] = mn;
+    __syncthreads();
+    for (int st = WG / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmin[blockIdx.x] = sm[0];
 }
 
-// find_Max_Min(op 1) (matching.cu:188-217): min |AH| (a min is order independent)
+// min over the blocks' minima into *out: non-negative doubles order like their bit patterns
 __global__ __launch_bounds__(WG) void k_absmin(long long m, const double* __restrict__ v, unsigned long long* __restrict__ out) {
     __shared__ double s[WG];
     double mn = DBL_MAX;
-    for (long long t = (long long)blockIdx.x * WG + threadIdx.x; t < m; t += (long long)gridDim.x * WG) mn = fmin(mn, fabs(v[t]));
+    for (long long t = (long long)blockIdx.x * WG + threadIdx.x; t < m; t += (long long)gridDim.x * WG) mn = fmin(mn, v[t]);
     s[threadIdx.x] = mn;
     __syncthreads();
     for (int st = WG / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + st]);
         __syncthreads();
     }
-    // non-negative doubles order like their bit patterns
     if (threadIdx.x == 0) atomicMin(out, (unsigned long long)__double_as_longlong(s[0]));
 }
 
 // _make_w (matching.cu:219-229) with toMaximumProductMatrix's clamp (:241-245)
-__global__ void k_make_W(long long m, double* __restrict__ v, const unsigned long long* __restrict__ minbits) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    double mn = __longlong_as_double((long long)*minbits);
+__device__ __forceinline__ double weight_of(double v, double mn) {
     if (fabs(mn) < DBL_EPSILON) mn = DBL_EPSILON;
-    double s = fabs(v[t]);
-    v[t] = log(s != 0.0 ? s : SUITOR_EPS / (0.999 * mn));
+    double s = fabs(v);
+    return log(s != 0.0 ? s : SUITOR_EPS / (0.999 * mn));
 }
 
 __device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
@@ -173,25 +183,31 @@ __device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
     return lo;
 }
 
-// _write_T_warp (matching.cu:29-64): W(i, c) <- W(c, i) for c < i.  The
+// _make_w (matching.cu:219-229) then _write_T_warp (matching.cu:29-64),
+// W(i, c) <- W(c, i) for c < i, in one pass from AH into a new array W.  The
 // mirrored pair (i, c) / (c, i) found here is recorded both ways in mir
 // (filled with -1 beforehand: entries without a mirror keep it).
 template <int L>
 __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                                   double* __restrict__ Wval, int* __restrict__ mir) {
+                                                   const double* __restrict__ AH, double* __restrict__ W,
+                                                   int* __restrict__ mir, const unsigned long long* __restrict__ minbits) {
     int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
     int lane = (int)threadIdx.x % L;
     if (i >= n) return;
+    double mn = __longlong_as_double((long long)*minbits);
     for (int j = Wrow[i] + lane; j < Wrow[i + 1]; j += L) {
         int cc = Wcol[j];
-        if (cc >= i) break;
-        int b = Wrow[cc], len = Wrow[cc + 1] - b;
-        int p = lower_bound_dev(Wcol + b, len, i);
-        if (p < len && Wcol[b + p] == i) {  // mirrored entry present
-            Wval[j] = Wval[b + p];
-            mir[j] = b + p;
-            mir[b + p] = j;
+        int src = j;
+        if (cc < i) {
+            int b = Wrow[cc], len = Wrow[cc + 1] - b;
+            int p = lower_bound_dev(Wcol + b, len, i);
+            if (p < len && Wcol[b + p] == i) {  // mirrored entry present
+                src = b + p;
+                mir[j] = b + p;
+                mir[b + p] = j;
+            }
         }
+        W[j] = weight_of(AH[src], mn);
     }
 }
 
@@ -459,31 +475,30 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     k_make_wrow<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, A.row.get(), Wrow.get());
     BCMGX_LAUNCHED();
     int L = lanes_for(A.nnz, n);
+    DBuf<unsigned long long> mn(c, 1);
+    double big = DBL_MAX;
+    long long bigbits;
+    std::memcpy(&bigbits, &big, sizeof(big));
+    fill<long long>(c, (long long*)mn.get(), 1, bigbits);
+    unsigned nb_ah = grid_for(n, WG / L);
+    DBuf<double> bmin(c, nb_ah);
 #define BCMGX_AH(LL)                                                                                               \
     case LL:                                                                                                       \
-        k_make_AH<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(),      \
-                                                                  dind.get(), w, C.get(), Wcol.get(), Wval.get()); \
+        k_make_AH<LL><<<nb_ah, WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(), dind.get(), w,      \
+                                                   C.get(), Wcol.get(), Wval.get(), bmin.get());                   \
         break;
     switch (L) { BCMGX_AH(4) BCMGX_AH(8) BCMGX_AH(16) BCMGX_AH(32) }
 #undef BCMGX_AH
     BCMGX_LAUNCHED();
-    DBuf<unsigned long long> mn(c, 1);
-    double big = DBL_MAX;
-    unsigned long long bigbits;
-    std::memcpy(&bigbits, &big, sizeof(big));
-    BCMGX_HIP(hipMemcpyAsync(mn.get(), &bigbits, sizeof(bigbits), hipMemcpyHostToDevice, c->stream));
-    if (m > 0) {
-        unsigned g = (unsigned)std::min<long long>(grid_for(m, WG), 2048);
-        k_absmin<<<g, WG, 0, c->stream>>>(m, Wval.get(), mn.get());
-        BCMGX_LAUNCHED();
-        k_make_W<<<grid_for(m, WG), WG, 0, c->stream>>>(m, Wval.get(), mn.get());
-        BCMGX_LAUNCHED();
-    }
+    k_absmin<<<(unsigned)std::min<long long>(grid_for(nb_ah, WG), 256), WG, 0, c->stream>>>(nb_ah, bmin.get(), mn.get());
+    BCMGX_LAUNCHED();
     DBuf<int> mir(c, mm);
     fill<int>(c, mir.get(), (long long)mm, -1);
-#define BCMGX_SY(LL)                                                                                                     \
-    case LL:                                                                                                             \
-        k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), mir.get()); \
+    DBuf<double> Wv(c, mm);  // the weights, from AH (Wval)
+#define BCMGX_SY(LL)                                                                                              \
+    case LL:                                                                                                      \
+        k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), Wv.get(), \
+                                                                     mir.get(), mn.get());                        \
         break;
     switch (L) { BCMGX_SY(4) BCMGX_SY(8) BCMGX_SY(16) BCMGX_SY(32) }
 #undef BCMGX_SY
@@ -499,9 +514,9 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     BCMGX_LAUNCHED();
 #define BCMGX_MT(LL)                                                                                                   \
     case LL:                                                                                                           \
-        k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get());    \
+        k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wv.get(), rank.get());      \
         BCMGX_LAUNCHED();                                                                                              \
-        k_pref<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), rank.get(),     \
+        k_pref<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wv.get(), rank.get(),       \
                                                                mir.get(), pref.get());                                 \
         BCMGX_LAUNCHED();                                                                                              \
         break;
