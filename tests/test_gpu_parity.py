@@ -195,6 +195,31 @@ def test_spgemm_medium_and_large_rows(ctx):
     _assert_same_csr(dev(ctx, E).spgemm(dev(ctx, E)), O.spgemm(E, E))
 
 
+def test_spgemm_two_entry_rows_merge(ctx):
+    """Left-operand rows with at most two entries (the restriction's rows) are
+    merged from two sorted B rows; both orders of k1 < k2 and k1 == k2, rows of
+    0, 1 and 2 entries, B rows from empty to 300 entries, overlapping columns."""
+    rng = np.random.default_rng(11)
+    nb, m = 400, 350
+    lens = rng.integers(0, 40, nb)
+    lens[::37] = 300
+    brow = np.concatenate([[0], np.cumsum(lens)])
+    bcol = np.concatenate([np.sort(rng.choice(m, l, replace=False)) for l in lens]).astype(np.int32)
+    bval = rng.normal(size=len(bcol))
+    oB = O.Csr.from_arrays(nb, m, brow, bcol, bval)
+    na = 500
+    arow, acol = [0], []
+    for i in range(na):
+        t = i % 5
+        ks = [] if t == 0 else [int(rng.integers(nb))] if t == 1 else sorted(rng.choice(nb, 2, replace=False).tolist())
+        if t == 4:
+            ks = [int(rng.integers(nb))] * 2  # the same B row twice
+        acol += ks
+        arow.append(len(acol))
+    oA = O.Csr.from_arrays(na, nb, np.array(arow), np.array(acol, np.int32), rng.normal(size=len(acol)))
+    _assert_same_csr(dev(ctx, oA).spgemm(dev(ctx, oB)), O.spgemm(oA, oB))
+
+
 def test_kernel_timers(ctx):
     """The smoother and SpGEMM micro-benchmarks run their kernels and report sane figures."""
     oa = O.laplacian3d(27, 12, 12, 12)
