@@ -710,21 +710,28 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         a.stop = io ? xzero : nullptr;
         spmv<double, double>(c, SpOp::RES, a);  // r = b - A x
     }
-    if (s.fused) {  // delta0 = ||r||, and the FP32 cycle's entry from r in the same pass
-        fcg_norm_entry(c, s.fb, s.r.get(), 0, s.entry);
-        prec->apply(s.r.get(), s.v.get(), k, true, nullptr, true);
+    if (s.fused) fcg_norm_entry(c, s.fb, s.r.get(), 0, s.entry);  // delta0 = ||r||, and the FP32 cycle's entry
+    else fcg_norm(c, s.fb, s.r.get(), 0);                         // delta0 = ||r||
+    if (s.z32) {
+        // v = B r kept in FP32; w = A v with (r.v, w.v) in the fused product (its third
+        // dot, against r here, is not used)
+        prec->apply_entry_f32(s.zf.get(), k, nullptr);
+        SpmvArgs a;
+        a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        a.x = s.zf.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
+        BCMGX_CHECK(spmm_dots_z32(c, a, s.r.get(), s.r.get(), s.fb.partials, s.z32_blocks, nullptr), BCMGX_ERR_STATE,
+                    "fused FP32-z product not applicable");
+        fcg_init_scalars(c, s.fb, s.z32_blocks);
+        fcg_init_update_z32(c, s.fb, s.X.get(), s.r.get(), s.zf.get(), s.w.get(), s.d.get(), s.q.get(), s.entry);
     } else {
-        fcg_norm(c, s.fb, s.r.get(), 0);  // delta0 = ||r||
-        prec->apply(s.r.get(), s.v.get(), k, true, nullptr);
-    }
-    {
+        prec->apply(s.r.get(), s.v.get(), k, true, nullptr, s.fused);
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
         a.x = s.v.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         spmv<double, double>(c, SpOp::AX, a);  // w = A v
+        fcg_init_dots(c, s.fb, s.r.get(), s.v.get(), s.w.get());
+        fcg_init_update(c, s.fb, s.X.get(), s.r.get(), s.v.get(), s.w.get(), s.d.get(), s.q.get(), s.entry);
     }
-    fcg_init_dots(c, s.fb, s.r.get(), s.v.get(), s.w.get());
-    fcg_init_update(c, s.fb, s.X.get(), s.r.get(), s.v.get(), s.w.get(), s.d.get(), s.q.get(), s.entry);
     // ---- iterations, two at a time (the buffer roles alternate)
     int check = std::max(2, p.check_every + (p.check_every & 1));
     bool use_graph = p.use_graph != 0;

@@ -180,12 +180,18 @@ void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* 
     int KP = kpad(f.k);
     k_dots<2><<<f.nblk, WG, 0, c->stream>>>(f.n, f.k, KP, r, v, w, v, nullptr, nullptr, f.partials, nullptr);
     BCMGX_LAUNCHED();
-    k_init_scalars<<<f.k, WG, 0, c->stream>>>(f.partials, f.nblk, f.k, f.st);
+    fcg_init_scalars(c, f, f.nblk);
+}
+
+void fcg_init_scalars(Context* c, const FcgBufs& f, int nblk) {
+    k_init_scalars<<<f.k, WG, 0, c->stream>>>(f.partials, nblk, f.k, f.st);
     BCMGX_LAUNCHED();
 }
 
+// v in FP64, or the FP32 cycle result widened exactly
+template <typename VT>
 __global__ __launch_bounds__(WG) void k_init_update(int n, int k, const FcgState* __restrict__ st, double* __restrict__ x,
-                                                    double* __restrict__ r, const double* __restrict__ v,
+                                                    double* __restrict__ r, const VT* __restrict__ v,
                                                     const double* __restrict__ w, double* __restrict__ d,
                                                     double* __restrict__ q, FcgEntry e) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(WG) void k_init_update(int n, int k, const FcgState
     int i = (int)(t / k);
     int c = (int)(t - (long long)i * k);
     double th = st[c].theta;
-    double vv = v[t], ww = w[t];
+    double vv = (double)v[t], ww = w[t];
     double rr = r[t];
     if (!st[c].done) {
         x[t] += th * vv;      // my_axpby(v, x, theta, 1): y += a x  (basic.cu:114-130)
@@ -213,7 +219,15 @@ void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const d
                      const FcgEntry& e) {
     long long nk = (long long)f.n * f.k;
     if (!nk) return;
-    k_init_update<<<grid_for(nk, WG), WG, 0, c->stream>>>(f.n, f.k, f.st, x, r, v, w, d, q, e);
+    k_init_update<double><<<grid_for(nk, WG), WG, 0, c->stream>>>(f.n, f.k, f.st, x, r, v, w, d, q, e);
+    BCMGX_LAUNCHED();
+}
+
+void fcg_init_update_z32(Context* c, const FcgBufs& f, double* x, double* r, const float* z, const double* w, double* d,
+                         double* q, const FcgEntry& e) {
+    long long nk = (long long)f.n * f.k;
+    if (!nk) return;
+    k_init_update<float><<<grid_for(nk, WG), WG, 0, c->stream>>>(f.n, f.k, f.st, x, r, z, w, d, q, e);
     BCMGX_LAUNCHED();
 }
 

@@ -122,6 +122,11 @@ struct FcgEntry {
 void fcg_norm_entry(Context* c, const FcgBufs& f, const double* r, int mode, const FcgEntry& e);
 // alpha = r.v, beta = w.v -> delta = beta, theta = alpha / delta  (FCG.cu:56-86)
 void fcg_init_dots(Context* c, const FcgBufs& f, const double* r, const double* v, const double* w);
+// the initial scalars from the partials of nblk blocks (alpha, beta in slots 0, 1)
+void fcg_init_scalars(Context* c, const FcgBufs& f, int nblk);
+// fcg_init_update with v the FP32 cycle result z, widened exactly
+void fcg_init_update_z32(Context* c, const FcgBufs& f, double* x, double* r, const float* z, const double* w, double* d,
+                         double* q, const FcgEntry& e = FcgEntry());
 // x += theta v; r -= theta w  (FCG.cu:88-89); d = v, q = w
 void fcg_init_update(Context* c, const FcgBufs& f, double* x, double* r, const double* v, const double* w, double* d, double* q,
                      const FcgEntry& e = FcgEntry());
