@@ -413,10 +413,17 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
         if constexpr (std::is_same<V, float>::value) {
             // the coarsest-level sweeps run in FP64 on the FP64 operator
             long long nk = (long long)A.n * k;
-            convert_stop<float, double>(c, F, pr->cd_F.get(), nk, R.stop);
-            if (!xzero) convert_stop<float, double>(c, cur, pr->cd_X.get(), nk, R.stop);
-            double* res = jacobi_sweeps<double>(c, A, lv.lanesA, lv.M.get(), pr->cd_F.get(), pr->cd_X.get(), pr->cd_X2.get(),
-                                                p.relaxnumber_coarse, xzero, k, R.stop);
+            double* res;
+            if (xzero && p.relaxnumber_coarse >= 1) {  // widening and the first sweep from zero in one pass
+                jacobi_from_zero_f32in(c, A.n, k, F, lv.M.get(), pr->cd_F.get(), pr->cd_X.get(), R.stop);
+                res = jacobi_sweeps<double>(c, A, lv.lanesA, lv.M.get(), pr->cd_F.get(), pr->cd_X.get(), pr->cd_X2.get(),
+                                            p.relaxnumber_coarse - 1, false, k, R.stop);
+            } else {
+                convert_stop<float, double>(c, F, pr->cd_F.get(), nk, R.stop);
+                if (!xzero) convert_stop<float, double>(c, cur, pr->cd_X.get(), nk, R.stop);
+                res = jacobi_sweeps<double>(c, A, lv.lanesA, lv.M.get(), pr->cd_F.get(), pr->cd_X.get(), pr->cd_X2.get(),
+                                            p.relaxnumber_coarse, xzero, k, R.stop);
+            }
             convert_stop<double, float>(c, res, cur, nk, R.stop);
             return cur;
         } else {

@@ -72,6 +72,8 @@ template <typename V>
 void jacobi_from_zero(Context* c, int n, int k, const V* f, const V* D, V* x, const int* stop);
 // mixed precision: F = (float) r; X = F / D   (level-0 entry of the FP32 cycle)
 void jacobi_from_zero_f64in(Context* c, int n, int k, const double* r, const float* D, float* F, float* X, const int* stop);
+// F = (double) f and the first sweep from zero X = F / D, both in FP64
+void jacobi_from_zero_f32in(Context* c, int n, int k, const float* f, const double* D, double* F, double* X, const int* stop);
 // x_i = p_i * xc[agg_i] + x_i   (prolongation, OP 0 with alpha = beta = 1)
 template <typename V>
 void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const V* xc, V* x, const int* stop);

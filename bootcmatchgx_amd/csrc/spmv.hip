@@ -944,6 +944,24 @@ void jacobi_from_zero_f64in(Context* c, int n, int k, const double* r, const flo
     BCMGX_LAUNCHED();
 }
 
+// FP32 f widened to FP64 F and the first sweep from zero X = F / D in FP64
+// (convert_stop<float, double> then jacobi_from_zero<double>, in one pass)
+__global__ void k_jacobi0_f32in(long long nk, int k, const float* __restrict__ f, const double* __restrict__ D,
+                                double* __restrict__ F, double* __restrict__ X, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    double v = (double)f[t];
+    F[t] = v;
+    X[t] = v / D[t / k];
+}
+void jacobi_from_zero_f32in(Context* c, int n, int k, const float* f, const double* D, double* F, double* X, const int* stop) {
+    long long nk = (long long)n * k;
+    if (!nk) return;
+    k_jacobi0_f32in<<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, k, f, D, F, X, stop);
+    BCMGX_LAUNCHED();
+}
+
 template <typename V>
 __global__ void k_prolong(long long nk, int k, const int* __restrict__ agg, const V* __restrict__ pval,
                           const V* __restrict__ xc, V* __restrict__ x, const int* __restrict__ stop) {
