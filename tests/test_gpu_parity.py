@@ -499,7 +499,8 @@ def test_missing_diagonal_is_reported(ctx):
 
 
 @pytest.mark.parametrize("prec,k,guess", [("FP32", 32, "zero"), ("FP32", 32, "rand"), ("FP64", 5, "zero"),
-                                           ("FP64", 2, "rand"), ("FP32", 12, "neg0"), ("FP64", 64, "zero")])
+                                           ("FP64", 2, "rand"), ("FP32", 12, "neg0"), ("FP64", 64, "zero"),
+                                           ("FP32", 64, "rand"), ("FP32", 3, "zero")])
 def test_fcg_device_blocks_match_host_path(ctx, prec, k, guess):
     """FCG on column-major device blocks stages them straight into its buffers
     (tiled layout change, the residual of a zero guess taken as b, the waiting x
