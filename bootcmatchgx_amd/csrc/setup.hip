@@ -779,8 +779,12 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
     return (unsigned long long)__shfl_xor((long long)v, m, w);
 }
 
-// W lanes per row (W = 16, 32 or 64), products <= W: 64 / W rows per wave
-template <int W>
+// W lanes per row (W = 16, 32 or 64), products <= W: 64 / W rows per wave.
+// K32: B has fewer than 2^26 - 1 columns, so (column, position) packs into one
+// 32-bit key; the sort then moves one 32-bit word per lane and stage, and each
+// lane fetches its product's value by position afterwards (same order, same
+// sums as the 64-bit key with the value carried along).
+template <int W, bool K32>
 __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
                                                      const int* __restrict__ acol, const double* __restrict__ aval,
                                                      const int* __restrict__ brow, const int* __restrict__ bcol,
@@ -789,6 +793,7 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     constexpr int G = WG / W;  // rows per block
     __shared__ int pre_s[G][W + 1];
     __shared__ double v_s[G][W];
+    __shared__ double p_s[K32 ? G : 1][K32 ? W : 1];  // K32: products by position
     int gid = threadIdx.x / W, lane = threadIdx.x % W;
     int wl = threadIdx.x & 63;             // lane in the wave
     int gshift = wl - lane;                // first wave lane of this row's group
@@ -813,6 +818,7 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     if (lane == 0) pre_s[gid][0] = 0;
     __syncthreads();
     unsigned long long key = ~0ull;
+    unsigned key32 = ~0u;
     double v = 0.0;
     if (lane < total) {
         // owner: largest a with pre[a] <= lane
@@ -828,25 +834,44 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
         int cc = bcol[jb];
         v = aval[s + a] * bval[jb];
         key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)lane;
+        key32 = ((unsigned)cc << 6) | (unsigned)lane;
     }
-    // bitonic sort of W (key, v) pairs across the group
+    if constexpr (K32) {
+        p_s[gid][lane] = v;
+        // bitonic sort of W 32-bit keys across the group
 #pragma unroll
-    for (int kk = 2; kk <= W; kk <<= 1) {
+        for (int kk = 2; kk <= W; kk <<= 1) {
 #pragma unroll
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            unsigned long long pk = shfl_xor_u64(key, j, W);
-            double pv = __shfl_xor(v, j, W);
-            bool up = (lane & kk) == 0;
-            bool lower = (lane & j) == 0;
-            bool take = (lower == up) ? (pk < key) : (pk > key);
-            if (take) {
-                key = pk;
-                v = pv;
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                unsigned pk = (unsigned)__shfl_xor((int)key32, j, W);
+                bool up = (lane & kk) == 0;
+                bool lower = (lane & j) == 0;
+                bool take = (lower == up) ? (pk < key32) : (pk > key32);
+                if (take) key32 = pk;
+            }
+        }
+        __syncthreads();  // p_s complete
+        if (lane < total) v = p_s[gid][key32 & 63u];
+    } else {
+        // bitonic sort of W (key, v) pairs across the group
+#pragma unroll
+        for (int kk = 2; kk <= W; kk <<= 1) {
+#pragma unroll
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                unsigned long long pk = shfl_xor_u64(key, j, W);
+                double pv = __shfl_xor(v, j, W);
+                bool up = (lane & kk) == 0;
+                bool lower = (lane & j) == 0;
+                bool take = (lower == up) ? (pk < key) : (pk > key);
+                if (take) {
+                    key = pk;
+                    v = pv;
+                }
             }
         }
     }
     v_s[gid][lane] = v;
-    int colv = (int)(key >> 32);
+    int colv = K32 ? (int)(key32 >> 6) : (int)(key >> 32);
     int prevc = __shfl_up(colv, 1, W);
     bool head = lane < total && (lane == 0 || prevc != colv);
     unsigned long long hm = __ballot(head);
@@ -1046,9 +1071,12 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     DBuf<int> tcol(c, tsz);
     DBuf<double> tval(c, tsz);
     DBuf<int> uniq(c, (size_t)n + 1);
+    // (column, position < 64) packs into 32 bits below the all-ones key of the idle lanes
+    bool k32 = B.m < (1 << 26);
 #define BCMGX_SMALL(BIN, W)                                                                                                 \
     if (bc[BIN] > 0) {                                                                                                      \
-        k_spgemm_small<W><<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(                                                 \
+        auto kern = k32 ? k_spgemm_small<W, true> : k_spgemm_small<W, false>;                                               \
+        kern<<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(                                                              \
             bc[BIN], bins.get() + (size_t)(BIN) * n, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(),      \
             B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get());                                                   \
         BCMGX_LAUNCHED();                                                                                                   \
