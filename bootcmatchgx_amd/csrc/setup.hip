@@ -195,12 +195,16 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
     int lane = (int)threadIdx.x % L;
     if (i >= n) return;
     double mn = __longlong_as_double((long long)*minbits);
-    for (int j = Wrow[i] + lane; j < Wrow[i + 1]; j += L) {
+    int si = Wrow[i], ei = Wrow[i + 1];
+    for (int j = si + lane; j < ei; j += L) {
         int cc = Wcol[j];
         int src = j;
         if (cc < i) {
             int b = Wrow[cc], len = Wrow[cc + 1] - b;
-            int p = lower_bound_dev(Wcol + b, len, i);
+            // a symmetric stencil puts the mirror at the reflected position of a
+            // row of the same length: one probe, else a binary search
+            int p = len - 1 - (j - si);
+            if (len != ei - si || p < 0 || p >= len || Wcol[b + p] != i) p = lower_bound_dev(Wcol + b, len, i);
             if (p < len && Wcol[b + p] == i) {  // mirrored entry present
                 src = b + p;
                 mir[j] = b + p;
