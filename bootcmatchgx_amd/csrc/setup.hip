@@ -125,15 +125,17 @@ __global__ __launch_bounds__(WG) void k_make_AH(int n, const int* __restrict__ r
     double mn = DBL_MAX;
     if (i < n) {
         int s = row[i], e = row[i + 1], d = dind[i];
+        // the row's own C_i, w_i once; C_lo + C_hi and w_lo w_hi in either
+        // operand order are the same bits (IEEE + and * commute)
+        double Ci = C[i], wi = w[i];
         for (int j = s + lane; j < e; j += L) {
             if (j == d) continue;
             int cc = col[j];
             int o = j - i - (j > d ? 1 : 0);
-            int lo = cc < i ? cc : i, hi = cc < i ? i : cc;
-            double norm = C[lo] + C[hi];
+            double norm = C[cc] + Ci;
             double v;
             if (norm > DBL_EPSILON) {
-                double wt = w[lo] * w[hi];
+                double wt = w[cc] * wi;
                 v = 1. - ((2. * val[j] * wt) / norm);
             } else {
                 v = DBL_EPSILON;
@@ -184,83 +186,81 @@ __device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
 }
 
 // _make_w (matching.cu:219-229) then _write_T_warp (matching.cu:29-64),
-// W(i, c) <- W(c, i) for c < i, in one pass from AH into a new array W.  The
-// mirrored pair (i, c) / (c, i) found here is recorded both ways in mir
-// (filled with -1 beforehand: entries without a mirror keep it).
+// W(i, c) <- W(c, i) for c < i, in one pass from AH into a new array W, and
+// each row's preference ranks.  A mirrored pair (i, c) / (c, i) has one weight
+// for both entries; when that weight is positive -- only then can the pair
+// carry a proposal (find_Candidate, suitor.cu:91-94) -- the pair is recorded
+// both ways in mir (filled with -1 beforehand: other entries keep it).
+//
+// Preference ranks.  Vertex i orders its neighbours c by (W(i,c), c) -- the
+// Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
+// rank[e] = position of entry e of row i in that order (0: the least liked).
+// A row of at most L entries keeps one entry per lane, staged in LDS, where
+// the group reads every entry of the row by broadcast; a longer row reads its
+// weights back from W after the block's barrier (the group wrote them).
 template <int L>
 __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                                   const double* __restrict__ AH, double* __restrict__ W,
-                                                   int* __restrict__ mir, const unsigned long long* __restrict__ minbits) {
-    int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
-    int lane = (int)threadIdx.x % L;
-    if (i >= n) return;
+                                                   const double* __restrict__ AH, double* W, int* __restrict__ mir,
+                                                   int* __restrict__ rank, const unsigned long long* __restrict__ minbits) {
+    __shared__ double sw[WG];
+    __shared__ int sc[WG];
+    int tid = (int)threadIdx.x;
+    int i = (blockIdx.x * WG + tid) / L;
+    int lane = tid % L;
+    int gb = tid - lane;
     double mn = __longlong_as_double((long long)*minbits);
-    int si = Wrow[i], ei = Wrow[i + 1];
+    int si = 0, ei = 0;
+    if (i < n) {
+        si = Wrow[i];
+        ei = Wrow[i + 1];
+    }
+    int len = ei - si;
+    double wa = 0.0;  // this lane's first entry (the only one when len <= L)
+    int ca = 0;
     for (int j = si + lane; j < ei; j += L) {
         int cc = Wcol[j];
         int src = j;
         if (cc < i) {
-            int b = Wrow[cc], len = Wrow[cc + 1] - b;
+            int b = Wrow[cc], lc = Wrow[cc + 1] - b;
             // a symmetric stencil puts the mirror at the reflected position of a
             // row of the same length: one probe, else a binary search
-            int p = len - 1 - (j - si);
-            if (len != ei - si || p < 0 || p >= len || Wcol[b + p] != i) p = lower_bound_dev(Wcol + b, len, i);
-            if (p < len && Wcol[b + p] == i) {  // mirrored entry present
-                src = b + p;
-                mir[j] = b + p;
-                mir[b + p] = j;
-            }
+            int p = lc - 1 - (j - si);
+            if (lc != len || p < 0 || p >= lc || Wcol[b + p] != i) p = lower_bound_dev(Wcol + b, lc, i);
+            if (p < lc && Wcol[b + p] == i) src = b + p;  // mirrored entry present
         }
-        W[j] = weight_of(AH[src], mn);
+        double wv = weight_of(AH[src], mn);
+        W[j] = wv;
+        if (src != j && wv > 0) {
+            mir[j] = src;
+            mir[src] = j;
+        }
+        if (j == si + lane) {
+            wa = wv;
+            ca = cc;
+        }
     }
-}
-
-// Preference ranks.  Vertex y orders its neighbours u by (W(y,u), u) -- the
-// Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
-// rank[e] = position of entry e of row y in that order (0: the least liked).
-// G lanes per row, each ranking a strided subset of the row's entries.
-template <int G>
-__global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                             const double* __restrict__ Wval, int* __restrict__ rank) {
-    // a row of at most G entries: one entry per lane, staged in LDS, where the
-    // group reads every entry of the row by broadcast
-    __shared__ double sw[WG];
-    __shared__ int sc[WG];
-    int tid = (int)threadIdx.x;
-    int y = (blockIdx.x * WG + tid) / G;
-    int lane = tid % G;
-    int gb = tid - lane;
-    int s = 0, e = 0;
-    if (y < n) {
-        s = Wrow[y];
-        e = Wrow[y + 1];
-    }
-    int len = e - s;
-    bool va = y < n && len <= G && lane < len;
-    double wa = va ? Wval[s + lane] : 0.0;
-    int ca = va ? Wcol[s + lane] : 0;
     sw[tid] = wa;
     sc[tid] = ca;
     __syncthreads();
-    if (y >= n) return;
-    if (len <= G) {  // uniform over the group
-        if (va) {
+    if (i >= n) return;
+    if (len <= L) {  // uniform over the group
+        if (lane < len) {
             int r = 0;
             for (int b = 0; b < len; b++) {
                 double wb = sw[gb + b];
                 int cb = sc[gb + b];
                 r += (wb < wa || (wb == wa && cb < ca)) ? 1 : 0;
             }
-            rank[s + lane] = r;
+            rank[si + lane] = r;
         }
         return;
     }
-    for (int a = s + lane; a < e; a += G) {
-        double w = Wval[a];
+    for (int a = si + lane; a < ei; a += L) {
+        double w = W[a];
         int cw = Wcol[a];
         int r = 0;
-        for (int b = s; b < e; b++) {
-            double wb = Wval[b];
+        for (int b = si; b < ei; b++) {
+            double wb = W[b];
             r += (wb < w || (wb == w && Wcol[b] < cw)) ? 1 : 0;
         }
         rank[a] = r;
@@ -271,19 +271,20 @@ __global__ __launch_bounds__(WG) void k_rank(int n, const int* __restrict__ Wrow
 // y of u in u's own order (the reverse of rank: heaviest first, ties to the
 // higher id), with key = 1 + the rank of u among y's neighbours.  key = 0 marks
 // a neighbour u never proposes to: non-positive weight (find_Candidate,
-// suitor.cu:91-94) or no mirrored entry (u is unknown to y).
+// suitor.cu:91-94) or no mirrored entry (u is unknown to y); mir holds only
+// the pairs of positive weight.
 template <int G>
 __global__ __launch_bounds__(WG) void k_pref(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                             const double* __restrict__ Wval, const int* __restrict__ rank,
-                                             const int* __restrict__ mir, int2* __restrict__ pref) {
+                                             const int* __restrict__ rank, const int* __restrict__ mir,
+                                             int2* __restrict__ pref) {
     int u = (blockIdx.x * WG + (int)threadIdx.x) / G;
     int lane = (int)threadIdx.x % G;
     if (u >= n) return;
     int s = Wrow[u], e = Wrow[u + 1];
     for (int a = s + lane; a < e; a += G) {
         int y = Wcol[a];
-        int q = mir[a];  // u's entry in y's row, -1 when absent
-        int key = (Wval[a] > 0 && y >= 0 && y < n && q >= 0) ? rank[q] + 1 : 0;
+        int q = mir[a];  // u's entry in y's row, -1 when absent or not positive
+        int key = (y >= 0 && y < n && q >= 0) ? rank[q] + 1 : 0;
         pref[e - 1 - rank[a]] = make_int2(y, key);
     }
 }
@@ -499,15 +500,15 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     DBuf<int> mir(c, mm);
     fill<int>(c, mir.get(), (long long)mm, -1);
     DBuf<double> Wv(c, mm);  // the weights, from AH (Wval)
+    DBuf<int> rank(c, mm);
 #define BCMGX_SY(LL)                                                                                              \
     case LL:                                                                                                      \
         k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), Wv.get(), \
-                                                                     mir.get(), mn.get());                        \
+                                                                     mir.get(), rank.get(), mn.get());            \
         break;
     switch (L) { BCMGX_SY(4) BCMGX_SY(8) BCMGX_SY(16) BCMGX_SY(32) }
 #undef BCMGX_SY
     BCMGX_LAUNCHED();
-    DBuf<int> rank(c, mm);
     DBuf<int2> pref(c, mm);
     DBuf<int> pos(c, n);
     DBuf<long long> S(c, n);
@@ -518,10 +519,8 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     BCMGX_LAUNCHED();
 #define BCMGX_MT(LL)                                                                                                   \
     case LL:                                                                                                           \
-        k_rank<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wv.get(), rank.get());      \
-        BCMGX_LAUNCHED();                                                                                              \
-        k_pref<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wv.get(), rank.get(),       \
-                                                               mir.get(), pref.get());                                 \
+        k_pref<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), rank.get(), mir.get(),       \
+                                                               pref.get());                                            \
         BCMGX_LAUNCHED();                                                                                              \
         break;
     switch (L) { BCMGX_MT(4) BCMGX_MT(8) BCMGX_MT(16) BCMGX_MT(32) }
