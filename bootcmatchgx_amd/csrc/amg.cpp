@@ -140,19 +140,21 @@ int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
 // setMaxCoarseSize (AMG.cu:238-241), truncation included
 static int reference_max_coarse_size(long long n) { return (int)(40 * std::pow((double)n, (double)1. / 3.)); }
 
-static void check_diagonals(Context* c, const DCsr& A) {
+// A's diagonal positions, every row checked to have one
+static DBuf<int> check_diagonals(Context* c, const DCsr& A) {
     DBuf<int> dind(c, std::max(A.n, 1));
     diag_index(c, A.n, A.row.get(), A.col.get(), dind.get());
-    if (count_missing_diagonals(c, A.n, dind.get()) == 0) return;
+    if (count_missing_diagonals(c, A.n, dind.get()) == 0) return dind;
     std::vector<int> h(A.n);  // name the first offending row
     c->d2h(h.data(), dind.get(), A.n);
     for (int i = 0; i < A.n; i++)
         BCMGX_CHECK(h[i] >= 0, BCMGX_ERR_ARG, "row " + std::to_string(i) + " has no diagonal entry (required by the matching)");
+    return dind;
 }
 
 static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H) {
     BCMGX_CHECK(A0.n == A0.m, BCMGX_ERR_ARG, "the matrix must be square");
-    check_diagonals(c, A0);
+    DBuf<int> dind0 = check_diagonals(c, A0);
     int maxcs = p.max_coarse_size > 0 ? p.max_coarse_size : reference_max_coarse_size(A0.n);
     H.maxcs = maxcs;
     int sweeps = p.aggrsweeps + 1;
@@ -178,7 +180,7 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             int n = Ai->n;
             DBuf<int> M(c, n), agg_i(c, n);
             DBuf<double> pval_i(c, n);
-            suitor_match(c, *Ai, wi.get(), M.get());
+            suitor_match(c, *Ai, wi.get(), M.get(), Ai == &A0 ? dind0.get() : nullptr);
             int nc = make_prolongator(c, n, M.get(), wi.get(), agg_i.get(), pval_i.get());
             BCMGX_CHECK(nc > 0, BCMGX_ERR_NUMERIC, "coarsening produced an empty level");
             DCsr Pi, Ri, AP, Anew;

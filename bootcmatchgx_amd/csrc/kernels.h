@@ -190,7 +190,9 @@ void l1_diag(Context* c, int n, const int* row, const int* col, const double* va
 
 struct MatchWork;
 // suitor(): makeAH + toMaximumProductMatrix + _write_T_warp + Suitor (matching.cu:253-288)
-void suitor_match(Context* c, const DCsr& A, const double* w, int* M);
+// dind_checked: A's diagonal positions, already checked to be present in every
+// row (nullptr: found and checked here)
+void suitor_match(Context* c, const DCsr& A, const double* w, int* M, const int* dind_checked = nullptr);
 // makeP_GPU with canonical singleton order; returns the coarse size
 int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval);
 // R = P^T for a one-entry-per-row P given as (agg, pval): rows sorted by fine index

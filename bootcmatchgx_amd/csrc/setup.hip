@@ -463,17 +463,23 @@ int count_missing_diagonals(Context* c, int n, const int* dind) {
     return c->read1(miss.get());
 }
 
-void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
+void suitor_match(Context* c, const DCsr& A, const double* w, int* M, const int* dind_checked) {
     int n = A.n;
     if (n <= 0) return;
     long long m = (long long)A.nnz - n;  // off-diagonal entries (one diagonal per row, checked by the caller)
     size_t mm = (size_t)std::max<long long>(m, 1);
-    DBuf<int> dind(c, n);
-    diag_index(c, n, A.row.get(), A.col.get(), dind.get());
-    // makeAH requires a diagonal entry in every row (matching.cu:155)
-    BCMGX_CHECK(count_missing_diagonals(c, n, dind.get()) == 0, BCMGX_ERR_ARG, "matching: every row needs a diagonal entry");
+    DBuf<int> dind_own;
+    const int* dind = dind_checked;
+    if (!dind) {
+        dind_own.reset(c, n);
+        diag_index(c, n, A.row.get(), A.col.get(), dind_own.get());
+        // makeAH requires a diagonal entry in every row (matching.cu:155)
+        BCMGX_CHECK(count_missing_diagonals(c, n, dind_own.get()) == 0, BCMGX_ERR_ARG,
+                    "matching: every row needs a diagonal entry");
+        dind = dind_own.get();
+    }
     DBuf<double> C(c, n);
-    k_make_C<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.val.get(), dind.get(), w, C.get());
+    k_make_C<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.val.get(), dind, w, C.get());
     BCMGX_LAUNCHED();
     DBuf<int> Wrow(c, (size_t)n + 1), Wcol(c, mm);
     DBuf<double> Wval(c, mm);
@@ -489,7 +495,7 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M) {
     DBuf<double> bmin(c, nb_ah);
 #define BCMGX_AH(LL)                                                                                               \
     case LL:                                                                                                       \
-        k_make_AH<LL><<<nb_ah, WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(), dind.get(), w,      \
+        k_make_AH<LL><<<nb_ah, WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(), dind, w,            \
                                                    C.get(), Wcol.get(), Wval.get(), bmin.get());                   \
         break;
     switch (L) { BCMGX_AH(4) BCMGX_AH(8) BCMGX_AH(16) BCMGX_AH(32) }
