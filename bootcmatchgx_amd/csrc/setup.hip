@@ -20,6 +20,28 @@ namespace bcmgx {
 constexpr int RS_ROWS = 128;
 constexpr int RS_CAP = 4096;
 
+// dst[0, cnt) = src[0, cnt) by the block's RS_ROWS threads, each with
+// RS_UNROLL loads in flight (loads of clamped, always valid indices, stored
+// only when in range); a strided copy loop waits on every load before its
+// LDS store
+constexpr int RS_UNROLL = 8;
+template <typename T>
+__device__ __forceinline__ void stage_rows(const T* __restrict__ src, T* dst, int cnt) {
+    for (int t0 = 0; t0 < cnt; t0 += RS_ROWS * RS_UNROLL) {
+        T v[RS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RS_UNROLL; u++) {
+            int t = t0 + u * RS_ROWS + (int)threadIdx.x;
+            v[u] = src[t < cnt ? t : cnt - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < RS_UNROLL; u++) {
+            int t = t0 + u * RS_ROWS + (int)threadIdx.x;
+            if (t < cnt) dst[t] = v[u];
+        }
+    }
+}
+
 __global__ __launch_bounds__(RS_ROWS) void k_diag_index_s(int n, const int* __restrict__ row, const int* __restrict__ col,
                                                      int* __restrict__ dind) {
     __shared__ int sc[RS_CAP];
@@ -27,14 +49,18 @@ __global__ __launch_bounds__(RS_ROWS) void k_diag_index_s(int n, const int* __re
     int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
     int base = row[r0], cnt = row[rend] - base;
     bool staged = cnt <= RS_CAP;
-    if (staged)
-        for (int t = threadIdx.x; t < cnt; t += RS_ROWS) sc[t] = col[base + t];
+    if (staged) stage_rows(col + base, sc, cnt);
     __syncthreads();
     int i = r0 + threadIdx.x;
     if (i >= n) return;
-    int d = -1;
-    for (int j = row[i]; j < row[i + 1]; j++)
-        if ((staged ? sc[j - base] : col[j]) == i) { d = j; break; }
+    int d = -1, s = row[i], e = row[i + 1];
+    if (staged) {
+        for (int j = s; j < e; j++)
+            if (sc[j - base] == i) { d = j; break; }
+    } else {
+        for (int j = s; j < e; j++)
+            if (col[j] == i) { d = j; break; }
+    }
     dind[i] = d;
 }
 
@@ -45,13 +71,17 @@ __global__ __launch_bounds__(RS_ROWS) void k_abs_row_sum_s(int n, const int* __r
     int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
     int base = row[r0], cnt = row[rend] - base;
     bool staged = cnt <= RS_CAP;
-    if (staged)
-        for (int t = threadIdx.x; t < cnt; t += RS_ROWS) sv[t] = val[base + t];
+    if (staged) stage_rows(val + base, sv, cnt);
     __syncthreads();
     int i = r0 + threadIdx.x;
     if (i >= n) return;
     double t = 0.0;
-    for (int j = row[i]; j < row[i + 1]; j++) t += fabs(staged ? sv[j - base] : val[j]);
+    int s = row[i], e = row[i + 1];
+    if (staged) {
+        for (int j = s; j < e; j++) t += fabs(sv[j - base]);
+    } else {
+        for (int j = s; j < e; j++) t += fabs(val[j]);
+    }
     M[i] = t;
 }
 
@@ -77,11 +107,10 @@ __global__ __launch_bounds__(RS_ROWS) void k_l1_diag(int n, const int* __restric
     int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
     int base = row[r0], cnt = row[rend] - base;
     bool staged = cnt <= RS_CAP;
-    if (staged)
-        for (int t = threadIdx.x; t < cnt; t += RS_ROWS) {
-            sc[t] = col[base + t];
-            sv[t] = val[base + t];
-        }
+    if (staged) {
+        stage_rows(col + base, sc, cnt);
+        stage_rows(val + base, sv, cnt);
+    }
     __syncthreads();
     int i = r0 + threadIdx.x;
     if (i >= n) return;
@@ -734,8 +763,7 @@ __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __
     int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
     int base = arow[r0], ne = arow[rend] - base;
     bool staged = ne <= RS_CAP;
-    if (staged)
-        for (int t = threadIdx.x; t < ne; t += RS_ROWS) sc[t] = acol[base + t];
+    if (staged) stage_rows(acol + base, sc, ne);
     __syncthreads();
     int i = r0 + threadIdx.x;
     if (i >= n) return;
