@@ -778,38 +778,54 @@ __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __
 // bins: 0..2 rows of <= 16 / 32 / 64 products (W-lane sorts), 3 medium, 4 large
 constexpr int NBINS = 5;
 // bin rows by product count (order inside a bin is irrelevant): the waves of a
-// block take slots from LDS counters, then one global atomic per non-empty
-// bin per block reserves the block's range
+// block take slots from LDS counters over BIN_CHUNKS consecutive chunks of WG
+// rows, then one global atomic per non-empty bin per block reserves the
+// block's range (all blocks add to the same few counters, so fewer, larger
+// blocks contend less)
+constexpr int BIN_CHUNKS = 8;
 __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ arow, const long long* __restrict__ cnt,
                                                  int* __restrict__ bins, int* __restrict__ bincount, int cap) {
     __shared__ int lcount[NBINS];
     __shared__ int lbase[NBINS];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     int lane = threadIdx.x & 63;
     if (threadIdx.x < NBINS) lcount[threadIdx.x] = 0;
-    __syncthreads();
-    int b = -1;
-    if (i < n) {
-        long long p = cnt[i];
-        int alen = arow[i + 1] - arow[i];
-        long long m = p > alen ? p : alen;
-        b = m <= 16 ? 0 : m <= 32 ? 1 : m <= SMALL_P ? 2 : (m <= MED_P ? 3 : 4);
+    long long i0 = (long long)blockIdx.x * BIN_CHUNKS * WG + threadIdx.x;
+    int bq[BIN_CHUNKS];
+#pragma unroll
+    for (int q = 0; q < BIN_CHUNKS; q++) {  // every chunk's loads first
+        long long i = i0 + (long long)q * WG;
+        int b = -1;
+        if (i < n) {
+            long long p = cnt[i];
+            int alen = arow[i + 1] - arow[i];
+            long long m = p > alen ? p : alen;
+            b = m <= 16 ? 0 : m <= 32 ? 1 : m <= SMALL_P ? 2 : (m <= MED_P ? 3 : 4);
+        }
+        bq[q] = b;
     }
+    __syncthreads();
     unsigned long long below = (1ull << lane) - 1ull;
-    int myslot = 0;
-    for (int bb = 0; bb < NBINS; bb++) {
-        unsigned long long msk = __ballot(b == bb);
-        if (!msk) continue;
-        int leader = __ffsll((long long)msk) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&lcount[bb], __popcll(msk));
-        base = __shfl(base, leader, 64);
-        if (b == bb) myslot = base + __popcll(msk & below);
+    int slot[BIN_CHUNKS];
+#pragma unroll
+    for (int q = 0; q < BIN_CHUNKS; q++) {
+        int myslot = 0;
+        for (int bb = 0; bb < NBINS; bb++) {
+            unsigned long long msk = __ballot(bq[q] == bb);
+            if (!msk) continue;
+            int leader = __ffsll((long long)msk) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&lcount[bb], __popcll(msk));
+            base = __shfl(base, leader, 64);
+            if (bq[q] == bb) myslot = base + __popcll(msk & below);
+        }
+        slot[q] = myslot;
     }
     __syncthreads();
     if (threadIdx.x < NBINS) lbase[threadIdx.x] = lcount[threadIdx.x] ? atomicAdd(&bincount[threadIdx.x], lcount[threadIdx.x]) : 0;
     __syncthreads();
-    if (b >= 0) bins[(long long)b * cap + lbase[b] + myslot] = i;
+#pragma unroll
+    for (int q = 0; q < BIN_CHUNKS; q++)
+        if (bq[q] >= 0) bins[(long long)bq[q] * cap + lbase[bq[q]] + slot[q]] = (int)(i0 + (long long)q * WG);
 }
 
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m, int w) {
@@ -1099,7 +1115,7 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     scan_exclusive_async(c, cnt.get(), poff.get(), n);
     DBuf<int> bins(c, (size_t)NBINS * n), bincount(c, NBINS);
     fill<int>(c, bincount.get(), NBINS, 0);
-    k_bin_rows<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), cnt.get(), bins.get(), bincount.get(), n);
+    k_bin_rows<<<grid_for(n, WG * BIN_CHUNKS), WG, 0, c->stream>>>(n, A.row.get(), cnt.get(), bins.get(), bincount.get(), n);
     BCMGX_LAUNCHED();
     long long total;
     int bc[NBINS];
