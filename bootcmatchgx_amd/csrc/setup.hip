@@ -249,15 +249,28 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
     for (int j = si + lane; j < ei; j += L) {
         int cc = Wcol[j];
         int src = j;
+        double ah;
         if (cc < i) {
             int b = Wrow[cc], lc = Wrow[cc + 1] - b;
             // a symmetric stencil puts the mirror at the reflected position of a
-            // row of the same length: one probe, else a binary search
+            // row of the same length: one probe, whose AH value is read with it
+            // (b + p < Wrow[cc + 1] <= j: in range), else a binary search
             int p = lc - 1 - (j - si);
-            if (lc != len || p < 0 || p >= lc || Wcol[b + p] != i) p = lower_bound_dev(Wcol + b, lc, i);
-            if (p < lc && Wcol[b + p] == i) src = b + p;  // mirrored entry present
+            bool inr = lc == len && p >= 0 && p < lc;
+            int q = inr ? b + p : j;
+            int wq = Wcol[q];
+            ah = AH[q];
+            if (inr && wq == i) {
+                src = q;  // mirrored entry present
+            } else {
+                p = lower_bound_dev(Wcol + b, lc, i);
+                if (p < lc && Wcol[b + p] == i) src = b + p;
+                ah = AH[src];
+            }
+        } else {
+            ah = AH[j];
         }
-        double wv = weight_of(AH[src], mn);
+        double wv = weight_of(ah, mn);
         W[j] = wv;
         if (src != j && wv > 0) {
             mir[j] = src;
