@@ -883,37 +883,64 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     pre_s[gid][lane + 1] = x;
     if (lane == 0) pre_s[gid][0] = 0;
     __syncthreads();
+    // A row of A with at most two entries (the rows of R in P^T (A P)) has at
+    // most two runs of products, each in column order when B's rows are.  Run 2
+    // then goes reversed into the top lanes (run 1 ascending, the padding,
+    // run 2 descending: a bitonic sequence), which the sort's last merge alone
+    // orders.  Keys carry their product position, so any layout sorts the same.
+    bool two = K32 && alen <= 2;  // uniform over the group
+    int n1 = two && alen == 2 ? pre_s[gid][1] : total, n2 = total - n1;
+    int pos = lane;  // the product this lane holds (W: padding)
+    if (two && lane >= n1) pos = lane >= W - n2 ? n1 + (W - 1 - lane) : W;
     unsigned long long key = ~0ull;
     unsigned key32 = ~0u;
     double v = 0.0;
-    if (lane < total) {
-        // owner: largest a with pre[a] <= lane
-        int lo = 0, hi = alen;  // pre[0] = 0 <= lane
+    if (pos < total) {
+        // owner: largest a with pre[a] <= pos
+        int lo = 0, hi = alen;  // pre[0] = 0 <= pos
         while (hi - lo > 1) {
             int mid = (lo + hi) >> 1;
-            if (pre_s[gid][mid] <= lane) lo = mid;
+            if (pre_s[gid][mid] <= pos) lo = mid;
             else hi = mid;
         }
         int a = lo;
         int ka = acol[s + a];
-        int jb = brow[ka] + (lane - pre_s[gid][a]);
+        int jb = brow[ka] + (pos - pre_s[gid][a]);
         int cc = bcol[jb];
         v = aval[s + a] * bval[jb];
-        key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)lane;
-        key32 = ((unsigned)cc << 6) | (unsigned)lane;
+        key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)pos;
+        key32 = ((unsigned)cc << 6) | (unsigned)pos;
     }
     if constexpr (K32) {
-        p_s[gid][lane] = v;
-        // bitonic sort of W 32-bit keys across the group
+        if (pos < total) p_s[gid][pos] = v;
+        bool merge_only = false;
+        if (two) {  // both runs in order (run 2 descending over the lanes)?
+            unsigned prv = (unsigned)__shfl_up((int)key32, 1, W);
+            bool bad = (lane >= 1 && lane < n1 && !(prv < key32)) || (lane > W - n2 && !(prv > key32));
+            unsigned long long bm = (__ballot(bad) >> gshift) & (W == 64 ? ~0ull : ((1ull << W) - 1ull));
+            merge_only = bm == 0;
+        }
+        if (merge_only) {
+            // the last merge of the bitonic sort (all lanes ascending)
 #pragma unroll
-        for (int kk = 2; kk <= W; kk <<= 1) {
-#pragma unroll
-            for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int j = W >> 1; j > 0; j >>= 1) {
                 unsigned pk = (unsigned)__shfl_xor((int)key32, j, W);
-                bool up = (lane & kk) == 0;
                 bool lower = (lane & j) == 0;
-                bool take = (lower == up) ? (pk < key32) : (pk > key32);
+                bool take = lower ? (pk < key32) : (pk > key32);
                 if (take) key32 = pk;
+            }
+        } else {
+            // bitonic sort of W 32-bit keys across the group
+#pragma unroll
+            for (int kk = 2; kk <= W; kk <<= 1) {
+#pragma unroll
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    unsigned pk = (unsigned)__shfl_xor((int)key32, j, W);
+                    bool up = (lane & kk) == 0;
+                    bool lower = (lane & j) == 0;
+                    bool take = (lower == up) ? (pk < key32) : (pk > key32);
+                    if (take) key32 = pk;
+                }
             }
         }
         __syncthreads();  // p_s complete
