@@ -174,13 +174,16 @@ __global__ __launch_bounds__(WG) void k_make_AH(int n, const int* __restrict__ r
             mn = fmin(mn, fabs(v));
         }
     }
-    sm[threadIdx.x] = mn;
+    // the min over each wave by shuffles, then over the block's waves
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mn;
     __syncthreads();
-    for (int st = WG / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + st]);
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = sm[0];
+        for (int q = 1; q < WG / 64; q++) b = fmin(b, sm[q]);
+        bmin[blockIdx.x] = b;
     }
-    if (threadIdx.x == 0) bmin[blockIdx.x] = sm[0];
 }
 
 // min over the blocks' minima into *out: non-negative doubles order like their bit patterns
