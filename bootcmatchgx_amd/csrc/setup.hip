@@ -784,7 +784,21 @@ __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __
     int i = r0 + threadIdx.x;
     if (i >= n) return;
     long long s = 0;
-    for (int j = arow[i]; j < arow[i + 1]; j++) {
+    int j = arow[i], je = arow[i + 1];
+    if (staged) {
+        // eight row-length gathers in flight (integer sums: any order is exact)
+        for (; j + 8 <= je; j += 8) {
+            int d[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                int k = sc[j + u - base];
+                d[u] = brow[k + 1] - brow[k];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += d[u];
+        }
+    }
+    for (; j < je; j++) {
         int k = staged ? sc[j - base] : acol[j];
         s += brow[k + 1] - brow[k];
     }
