@@ -98,9 +98,13 @@ class SparseSolve(torch.autograd.Function):
     def forward(fctx, values, b, system):
         if values.device.type != "cpu" or b.device.type != "cpu":
             raise ValueError("SparseSolve takes CPU tensors")
-        system.set_values(_np(values))
+        vals = _np(values)
+        system.set_values(vals)
         x = system.solve(_np(b))
         fctx.system = system
+        # the adjoint needs this forward's matrix: another forward on the same
+        # system may change the device values before backward runs
+        fctx.values = vals.copy()
         out = torch.from_numpy(np.ascontiguousarray(x)).to(b.dtype)
         fctx.save_for_backward(out)
         return out
@@ -109,6 +113,7 @@ class SparseSolve(torch.autograd.Function):
     def backward(fctx, gx):
         (x,) = fctx.saved_tensors
         system = fctx.system
+        system.set_values(fctx.values)
         # adjoint solve with the forward hierarchy: A^T = A
         lam = system.solve(_np(gx))
         g_b = torch.from_numpy(np.ascontiguousarray(lam)).to(gx.dtype) if fctx.needs_input_grad[1] else None

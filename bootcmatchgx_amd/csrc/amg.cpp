@@ -274,6 +274,10 @@ struct SolveWork {
     int lanesA = 0;
     int chunkA = 0;
     int mv_blocks = 0;  // grid of the fused SpMM + dots kernel (0: not used)
+    // the stop rule baked into the captured graphs (the finishing kernels take
+    // rtol and itnlim by value): a solve with another rule captures again
+    double cap_rtol = -1;
+    int cap_itnlim = -1;
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -483,6 +487,16 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
     return cur;
 }
 
+void Prec::refresh_level0() {
+    if (ptype == BCMGX_PREC_L1_JACOBI) {
+        l1_diag(ctx, A->n, A->row.get(), A->col.get(), A->val.get(), pl1j.get());
+    } else if (ptype == BCMGX_PREC_BCMG && !H.L.empty()) {
+        Level& l0 = *H.L[0];
+        abs_row_sum(ctx, A->n, A->row.get(), A->val.get(), l0.M.get());
+        if (l0.Mf) convert<double, float>(ctx, l0.M.get(), l0.Mf.get(), A->n);
+    }
+}
+
 bool Prec::fused_entry(int k, FcgEntry* e) {
     if (ptype != BCMGX_PREC_BCMG || p.precision != BCMGX_PRECISION_FP32 || H.levels() < 2 || p.prerelax_sweeps < 1)
         return false;
@@ -688,8 +702,9 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     double t0 = now_s();
     int n = A.n;
     SolveWork& s = solve_work(c, A, prec, p, k);
-    s.fb.rtol = p.rtol;
-    s.fb.itnlim = p.itnlim;
+    if (s.cap_rtol != p.rtol || s.cap_itnlim != p.itnlim) s.drop_graph();
+    s.fb.rtol = s.cap_rtol = p.rtol;
+    s.fb.itnlim = s.cap_itnlim = p.itnlim;
     prec->ensure_work(k);
     s.entry = FcgEntry();
     s.fused = prec->fused_entry(k, &s.entry);
@@ -816,8 +831,9 @@ SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params
     double t0 = now_s();
     int n = A.n;
     SolveWork& s = solve_work(c, A, prec, p, k);
-    s.fb.rtol = p.rtol;
-    s.fb.itnlim = p.itnlim;
+    if (s.cap_rtol != p.rtol || s.cap_itnlim != p.itnlim) s.drop_graph();
+    s.fb.rtol = s.cap_rtol = p.rtol;
+    s.fb.itnlim = s.cap_itnlim = p.itnlim;
     prec->ensure_work(k);
     long long nk = (long long)n * k;
     copy_d2d(c, s.B.get(), b, sizeof(double) * nk);

@@ -76,6 +76,9 @@ struct Prec {
     // the FP32 cycle from that entry, its result z left in FP32 (k interleaved
     // columns): the values the FP64 output would hold, before the exact widening
     void apply_entry_f32(float* z, int k, const int* stop);
+    // recompute what depends on level 0's values after they changed in place:
+    // the l1 row sums M (and Mf) of BCMG, the l1 diagonal of one-level l1-Jacobi
+    void refresh_level0();
 };
 
 std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& p);

@@ -17,8 +17,10 @@ struct bcmgx_matrix_s;
 struct bcmgx_prec_s;
 // A context owns the device memory of every handle created on it.  Destroying
 // the context releases that memory and orphans the handles (ctx = nullptr);
-// their own destroy calls then only free the host structures, so any
-// destruction order is safe.
+// their own destroy calls then only free the host structures.  Destroying a
+// matrix orphans the preconditioners set up on it the same way (their level 0
+// is that matrix).  Any destruction order is therefore safe, and an orphaned
+// handle answers BCMGX_ERR_STATE.
 struct bcmgx_context_s {
     Context* c = nullptr;
     std::set<bcmgx_matrix_s*> mats;
@@ -223,6 +225,18 @@ int bcmgx_csr_destroy(bcmgx_matrix A) {
     if (A->ctx) {
         (void)hipSetDevice(A->ctx->c->device);
         A->ctx->c->sync();
+        auto& precs = A->ctx->precs;
+        for (auto it = precs.begin(); it != precs.end();) {
+            bcmgx_prec_s* p = *it;
+            if (p->A == A) {
+                p->P.reset();
+                p->A = nullptr;
+                p->ctx = nullptr;
+                it = precs.erase(it);
+            } else {
+                ++it;
+            }
+        }
         A->ctx->mats.erase(A);
     }
     delete A;
@@ -250,6 +264,9 @@ int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem) {
         BCMGX_HIP(hipMemcpyAsync(M.val.get(), val, sizeof(double) * (size_t)M.nnz, k, c->stream));
         // an FP32 copy (built by an FP32 hierarchy on this matrix) follows the new values
         if (M.valf) convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
+        // so do the level-0 divisors of the preconditioners set up on it
+        for (auto* p : A->ctx->precs)
+            if (p->A == A && p->P) p->P->refresh_level0();
     }
     c->sync();
     return BCMGX_OK;
@@ -487,7 +504,7 @@ int bcmgx_dump_info(const char* path, const bcmgx_params* p, bcmgx_prec prec, co
     std::fprintf(fp, "Local dimension (number of rows)             : %d\n", out->local_n);
     std::fprintf(fp, "Initial residual                             : %.10lf\n", out->del0);
     std::fprintf(fp, "Final residual                               : %.10lf\n", out->exit_res);
-    if (p->preconditioner == BCMGX_PREC_BCMG && prec) {
+    if (p->preconditioner == BCMGX_PREC_BCMG && prec && prec->P) {
         Prec* P = prec->P.get();
         std::fprintf(fp, "Number of hierarchies                        : %d\n", 1);
         std::fprintf(fp, "Estimated ratio                              : %f\n", 1.0);

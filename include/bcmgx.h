@@ -125,6 +125,10 @@ const char* bcmgx_last_error(bcmgx_context ctx);
  */
 int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, const int* col,
                      const double* val, int mem, bcmgx_matrix* A);
+/* Destroying a matrix also releases the preconditioners set up on it (their
+ * level 0 is this matrix): their later calls return BCMGX_ERR_STATE, and
+ * bcmgx_prec_finalize still frees their handles.  Destroying the context
+ * releases every matrix and preconditioner on it the same way. */
 int bcmgx_csr_destroy(bcmgx_matrix A);
 int bcmgx_csr_info(bcmgx_matrix A, int* n, int* m, int* nnz);
 /* copy CSR arrays back to host buffers (row: n+1, col/val: nnz) */
@@ -132,8 +136,10 @@ int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val);
 /* Overwrite the nnz values of A (same pattern).  No counterpart in the
  * reference, whose matrices are immutable after copyToDevice (CSR.h:347);
  * used by the autograd shim when the values it differentiates change.  A
- * preconditioner set up on A keeps its hierarchy (its level 0 is A itself,
- * so it sees the new values; coarse levels are not rebuilt). */
+ * preconditioner set up on A keeps its hierarchy: its level 0 is A itself and
+ * sees the new values, and the level-0 divisors are recomputed from them (the
+ * l1 row sums of BCMG, the l1 diagonal of one-level l1-Jacobi); the coarse
+ * levels, the prolongators and an AFSAI factor are not rebuilt. */
 int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem);
 
 /* Host-side problem generators: src/generator/laplacian.cu:119-334 (7-point),

@@ -155,3 +155,20 @@ def test_new_values_with_rebuilt_hierarchy(ctx):
         assert rel(x, spla.spsolve(S2.tocsc(), b)) < 1e-9
     finally:
         system.close()
+
+
+def test_backward_uses_its_own_forward_values(c4):
+    """Two forwards on one system with different values, then the first one's
+    backward: its adjoint runs on the first forward's matrix, not the values
+    the second forward left on the device."""
+    s = c4
+    v1 = torch.tensor(s["val"], requires_grad=True)
+    rows = np.repeat(np.arange(s["n"]), np.diff(s["row"]))
+    val2 = np.where(rows == s["col"], s["val"], 0.8 * s["val"])
+    bt = torch.tensor(s["b"])
+    x1 = AG.sparse_solve(v1, bt, s["system"])
+    AG.sparse_solve(torch.tensor(val2), bt, s["system"])  # another matrix on the same system
+    (0.5 * ((x1 - torch.tensor(s["xt"])) ** 2).sum()).backward()
+    g_ref = -s["lam_ref"][s["rows"]] * s["x_ref"][s["col"]]
+    print(f"backward after a second forward: dL/dA {rel(v1.grad.numpy(), g_ref):.3e}")
+    assert rel(v1.grad.numpy(), g_ref) < 1e-6

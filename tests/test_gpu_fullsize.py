@@ -5,14 +5,22 @@ solver's recurrence) is at most the requested 1e-8, mixed precision agrees
 with the FP64 hierarchy, batched columns agree with one-at-a-time solves,
 and b = A x* recovers x*.
 
+The hierarchies of the 7-point 50^3 and 27-point 128^3 Laplacians are
+compared with the oracle's bit for bit (aggregates, prolongator values,
+coarse operators, complexities).
+
 Workloads: the 27-point Laplacian on 128^3 (2,097,152 rows, 55,742,968
 nonzeros) with x* ~ U[-1,1] (numpy PCG64 seed 1) and with k = 1, 8, 32, 64
 right-hand sides N(0,1) (seed 3); the 7-point diffusion operator on 160^3
 with per-face conductivities exp(2 g), g ~ N(0,1) (seed 2)."""
+import ctypes
+
 import numpy as np
 import pytest
 
 import bootcmatchgx_amd as bx
+import oracle_py as O
+from bootcmatchgx_amd import abi
 
 pytestmark = pytest.mark.gpu
 
@@ -126,3 +134,39 @@ def test_c2_fp32_hierarchy_matches_fp64(ctx):
     # limit and the measured difference is printed
     assert err < 1e-3
     A.close()
+
+
+@pytest.mark.parametrize("pts,m", [(7, 50), (27, 128)])
+def test_config_hierarchy_bit_exact(ctx, lap27, pts, m):
+    """BCMG setup at the configurations' sizes against the oracle's, bit for bit."""
+    if pts == 27 and m == 128:
+        A = lap27
+    else:
+        n, row, col, val = bx.laplacian3d(pts, m, m, m)
+        A = bx.CSR(ctx, n, n, row, col, val)
+        del row, col, val
+    oa = O.laplacian3d(pts, m, m, m)
+    p = params()
+    q = abi.Params()
+    ctypes.pointer(q)[0] = p.s
+    P = bx.prec_setup(ctx, A, p)
+    OP = O.Prec(oa, q)
+    nl = P.levels()
+    print(f"{pts}p {m}^3: {nl} levels, sizes {[P.level_size(l) for l in range(nl)]}")
+    assert nl == OP.levels()
+    for l in range(nl):
+        if l > 0:
+            g = P.level_matrix(l)
+            o = OP.level_A(l).to_scipy()
+            assert np.array_equal(g.indptr, o.indptr) and np.array_equal(g.indices, o.indices), l
+            assert np.array_equal(g.data, o.data), l
+        if l < nl - 1:
+            a1, p1 = P.level_prolongator(l)
+            a2, p2 = OP.level_P(l)
+            assert np.array_equal(a1, a2) and np.array_equal(p1, p2), l
+    info, st = P.info(), OP.stats()
+    assert info["max_coarse_size"] == st["max_coarse_size"]
+    assert info["op_cmplx"] == st["op_cmplx"] and info["avg_cratio"] == st["avg_cratio"]
+    P.finalize()
+    if A is not lap27:
+        A.close()
