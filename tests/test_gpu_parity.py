@@ -129,11 +129,34 @@ def test_matching_chains_and_long_rows(ctx, name):
         assert np.array_equal(A.matching(w), O.suitor(oa, w)), seed
 
 
-def test_matching_bcsstk09(ctx, golden):
-    oa = O.read_mtx(os.path.join(golden, "bcsstk09.mtx"))
+@pytest.mark.parametrize("name", ["bcsstk09", "bcsstk13", "bcsstk24"])
+def test_matching_harwell_boeing(ctx, golden, name):
+    """Structural-engineering SPD matrices of the reference's test data: untied weights."""
+    oa = O.read_mtx(os.path.join(golden, name + ".mtx"))
     A = dev(ctx, oa)
     w = np.ones(A.n)
     assert np.array_equal(A.matching(w), O.suitor(oa, w))
+
+
+@pytest.mark.parametrize("name", ["bcsstk13", "bcsstk24"])
+def test_hierarchy_bit_exact_harwell_boeing(ctx, golden, name):
+    """The whole BCMG setup on the reference's bcsstk13/24 inputs (their solution files
+    hold zeros, so they pin the setup, not a solve): levels, aggregates, prolongator
+    values and coarse operators equal the oracle's bit for bit."""
+    oa = O.read_mtx(os.path.join(golden, name + ".mtx"))
+    A = dev(ctx, oa)
+    p = params("BCMG")
+    P = bx.prec_setup(ctx, A, p)
+    OP = O.Prec(oa, oparams(p))
+    assert P.levels() == OP.levels() >= 2
+    for l in range(P.levels()):
+        r1, r2 = P.level_matrix(l), OP.level_A(l).to_scipy()
+        assert np.array_equal(r1.indptr, r2.indptr) and np.array_equal(r1.indices, r2.indices)
+        assert np.array_equal(r1.data, r2.data), l
+        if l < P.levels() - 1:
+            a1, p1 = P.level_prolongator(l)
+            a2, p2 = OP.level_P(l)
+            assert np.array_equal(a1, a2) and np.array_equal(p1, p2), l
 
 
 def test_make_prolongator_exact(ctx):

@@ -81,7 +81,8 @@ def test_enum_strings():
             assert bx.lib().bcmgx_enum_from_string(kind.encode(), name.encode()) == v
 
 
-@pytest.mark.parametrize("args", [(7, 9, 8, 7, 1, 1, 1), (27, 6, 5, 4, 1, 1, 1), (7, 2, 2, 5, 3, 4, 2), (27, 5, 5, 5, 1, 2, 2)])
+@pytest.mark.parametrize("args", [(7, 9, 8, 7, 1, 1, 1), (27, 6, 5, 4, 1, 1, 1), (7, 2, 2, 5, 3, 4, 2), (27, 5, 5, 5, 1, 2, 2),
+                                  (27, 5, 5, 5, 4, 2, 1), (7, 4, 14, 4, 2, 2, 1)])
 def test_host_generator_equals_oracle(args):
     n, row, col, val = bx.laplacian3d(*args)
     r2, c2, v2 = O.laplacian3d(*args).arrays()

@@ -31,6 +31,8 @@ def mtx(path):
 @pytest.mark.parametrize("fname,args", [
     ("lap3d_7p_2x2x5_3x4x2.mtx", (7, 2, 2, 5, 3, 4, 2)),
     ("lap3d_27p_5x5x5_1x2x2.mtx", (27, 5, 5, 5, 1, 2, 2)),
+    ("lap3d_27p_5x5x5_4x2x1.mtx", (27, 5, 5, 5, 4, 2, 1)),
+    ("lap3d_7p_4x14x4_2x2x1.mtx", (7, 4, 14, 4, 2, 2, 1)),
 ])
 def test_generator_matches_matlab_fixture(golden, fname, args):
     """laplacian.cu block generators vs the reference's MATLAB-made matrices."""
