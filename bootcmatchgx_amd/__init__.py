@@ -84,6 +84,8 @@ def lib():
         "bcmgx_spmv": (C.c_int, [_VP, _VP, _VP, _VP, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
         "bcmgx_spmm": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                  C.c_int]),
+        "bcmgx_spmm_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                         C.c_int]),
         "bcmgx_spgemm": (C.c_int, [_VP, _VP, _VP, C.POINTER(_VP)]),
         "bcmgx_transpose": (C.c_int, [_VP, _VP, C.POINTER(_VP)]),
         "bcmgx_matching": (C.c_int, [_VP, _VP, _F64P, _I32P]),

@@ -77,6 +77,14 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     return best;
 }
 
+int cached_lanes(Context* c, const DCsr& A, bool fp32) {
+    if (A.n >= 65536) {
+        auto it = c->tuned_lanes.find(std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d"));
+        if (it != c->tuned_lanes.end()) return it->second;
+    }
+    return default_lanes(A.nnz, A.n);
+}
+
 template <typename V>
 static float time_spmm(Context* c, const DCsr& A, const void* val, int k, int chunk, hipEvent_t e0, hipEvent_t e1) {
     DBuf<V> x(c, (size_t)A.m * k), y(c, (size_t)A.n * k);

@@ -110,6 +110,8 @@ SolveResult cgs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
 
 // autotune: fastest lanes per row for SpMV with A (k = 1)
 int tune_lanes(Context* c, const DCsr& A, bool fp32);
+// the lanes tune_lanes chose for A if it ran, else the nnz/row heuristic (no timing, no allocation)
+int cached_lanes(Context* c, const DCsr& A, bool fp32);
 // autotune: fastest column chunk for SpMM with A on k interleaved columns
 int tune_chunk(Context* c, const DCsr& A, bool fp32, int k);
 

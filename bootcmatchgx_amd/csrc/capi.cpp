@@ -648,6 +648,19 @@ int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, in
     API_END(ctx)
 }
 
+int bcmgx_spmm_noalloc(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
+                       double beta, int op, int kchunk) {
+    if (!ctx || !A || !x || !y || op < 0 || op > 2 || k < 1 || k > 64 || (kchunk < -2 && (kchunk > -16 || kchunk < -28)))
+        return BCMGX_ERR_ARG;
+    if (A->ctx != ctx) return BCMGX_ERR_STATE;
+    API_BEGIN
+    Context* c = ctx->c;
+    BCMGX_HIP(hipSetDevice(c->device));
+    spmm_noalloc(c, A->A, x, y, k, ld, alpha, beta, op, kchunk, k == 1 ? cached_lanes(c, A->A, false) : 0);
+    return BCMGX_OK;
+    API_END(ctx)
+}
+
 int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int lanes, int reps, double* ms) {
     if (!ctx || !A || !ms || k < 1 || k > 64 || reps < 1 || (precision != 0 && precision != 1)) return BCMGX_ERR_ARG;
     API_BEGIN

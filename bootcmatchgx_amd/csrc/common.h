@@ -112,6 +112,8 @@ struct Context {
     size_t pinned_bytes = 0;
     // autotune cache: matrix signature -> lanes per row
     std::map<std::string, int> tuned_lanes;
+    // this device's copy of the fixed scratch array of the allocation-free entry point
+    void* scratch = nullptr;
 
     explicit Context(int dev);
     ~Context();

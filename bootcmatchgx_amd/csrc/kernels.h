@@ -6,6 +6,16 @@
 
 namespace bcmgx {
 
+// ---- allocation-free SpMM (noalloc.hip) ------------------------------------------
+// device address of `device`'s copy of the fixed scratch array (taken once per
+// device; `device` must be current), and the array's size in bytes
+void* scratch_address(int device);
+size_t scratch_bytes();
+// bcmgx_spmm on device blocks with both operands staged in c->scratch
+// (no allocation); lanes: lanes per row for k == 1
+void spmm_noalloc(Context* c, const DCsr& A, const double* x, double* y, int k, int ld, double alpha, double beta, int op,
+                  int kchunk, int lanes);
+
 // ---- primitives (prims.hip) -------------------------------------------------
 // out[0..n) = exclusive prefix sums of in, out[n] = total; returns the total.
 int scan_exclusive(Context* c, const int* in, int* out, long long n);

@@ -76,6 +76,7 @@ Context::Context(int dev) : device(dev) {
     hipDeviceProp_t prop;
     BCMGX_HIP(hipGetDeviceProperties(&prop, dev));
     num_cus = prop.multiProcessorCount;
+    scratch = scratch_address(dev);
 }
 
 Context::~Context() {

@@ -249,6 +249,20 @@ int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, do
  * (the oracle's order) bit for bit. */
 int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
                double beta, int op, int kchunk, int mem);
+/* The same product without allocating: x and y are device blocks, and the two
+ * operands are staged in the library's fixed per-device scratch array of
+ * 64 MiB ((rows + columns) x nrhs x 8 bytes, each part rounded up to 256
+ * bytes, must fit; a larger call returns BCMGX_ERR_ARG with the sizes in the
+ * message).  No counterpart in the reference: it serves callers that cannot
+ * allocate device memory, such as code under HIP graph capture and tight
+ * serving loops.  It calls no hipMalloc/hipFree and does not autotune (k = 1
+ * uses the lanes per row bcmgx_spmv/bcmgx_spmm tuned for A, if they ran, else
+ * the nnz-per-row heuristic; kchunk as in bcmgx_spmm).  It returns once the
+ * work is queued on the context's stream; bcmgx_context_synchronize waits for
+ * it.  Every call on one device shares that device's array: calls that
+ * overlap on one device (the streams of two contexts, two host threads) race. */
+int bcmgx_spmm_noalloc(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, int k, int ld, double alpha,
+                       double beta, int op, int kchunk);
 /* C = A*B (replaces nsparse spgemm_csrseg_kernel_hash, kernel_spgemm_hash_d.cu:1196-1247),
  * sorted columns; each C_ij summed over k in ascending order. */
 int bcmgx_spgemm(bcmgx_context ctx, bcmgx_matrix A, bcmgx_matrix B, bcmgx_matrix* C);
