@@ -89,18 +89,10 @@ def pmc_traffic(name="spmv_fp64_lap27_128.txt"):
     return None
 
 
-def cpu_baseline(size, nrhs, rtol):
-    """The oracle (test infrastructure) timed on the host: FP64 setup + one solve."""
-    import numpy as np
-
+def oracle_params(rtol):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py as O
     from bootcmatchgx_amd import abi
 
-    threads = O.lib().or_num_threads()
-    A = O.laplacian3d(27, size, size, size)
-    n = A.dims()[0]
-    b = np.random.default_rng(3).normal(size=n)
     p = abi.reference_defaults()
     p.solver = abi.SOLVER["FCG"]
     p.preconditioner = abi.PREC["BCMG"]
@@ -108,20 +100,155 @@ def cpu_baseline(size, nrhs, rtol):
     p.prerelax_sweeps = 1
     p.postrelax_sweeps = 1
     p.dispnorm = 0
+    return p
+
+
+def cpu_baseline(size, nrhs, rtol, samples=5):
+    """The oracle (test infrastructure) timed on the host: FP64 setup, then FCG solves
+    of the first `samples` columns; value = setup + nrhs x the median solve."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+
+    threads = O.lib().or_num_threads()
+    A = O.laplacian3d(27, size, size, size)
+    n = A.dims()[0]
+    Bh = np.random.default_rng(3).normal(size=(n, samples))
+    p = oracle_params(rtol)
     t0 = time.perf_counter()
     P = O.Prec(A, p)
-    t1 = time.perf_counter()
-    _, info = O.fcg(A, P, p, b)
-    t2 = time.perf_counter()
-    setup, solve = t1 - t0, t2 - t1
+    setup = time.perf_counter() - t0
+    solves, its = [], []
+    for j in range(samples):
+        t1 = time.perf_counter()
+        _, info = O.fcg(A, P, p, np.ascontiguousarray(Bh[:, j]))
+        solves.append(time.perf_counter() - t1)
+        its.append(info["niter"])
+    med = float(np.median(solves))
     return {
-        "value": setup + nrhs * solve,
+        "value": setup + nrhs * med,
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": f"27p {size}^3 FP64 BCMG setup ({setup:.2f} s) + one FCG solve of column 0 "
-                  f"({solve:.2f} s, {info['niter']} iterations); value = setup + {nrhs} x solve",
+        "sample": f"27p {size}^3 FP64 BCMG setup ({setup:.2f} s) + FCG solves of columns 0..{samples - 1} "
+                  f"({', '.join(f'{t:.2f}' for t in solves)} s; {its} iterations); "
+                  f"value = setup + {nrhs} x the median solve ({med:.2f} s)",
+        "setup_s": setup,
+        "solve_s": solves,
     }
+
+
+def timed_solve(bx, ctx, A, B, X, Z, k, p, prec_params=None):
+    """setup + solve of k device-resident columns from a zero guess; seconds and SolverOut."""
+    X.copy_from(Z)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    P = bx.prec_setup(ctx, A, prec_params or p)
+    t1 = time.perf_counter()
+    out = bx.solve_device(ctx, A, B.ptr, X.ptr, k, p, P)
+    t2 = time.perf_counter()
+    P.finalize()
+    return {"time_s": t2 - t0, "setup_s": t1 - t0, "solve_s": t2 - t1, "niter": out["niter"],
+            "rel_res": out["rel_res"]}
+
+
+def config_measurements(bx, ctx, A128, n128, steps):
+    """One line per named configuration besides the headline (each solve from a zero
+    guess to relative residual 1e-8, inputs resident in HBM, one untimed warm-up run
+    that also tunes the kernels, then the mean of `steps` timed runs)."""
+    import numpy as np
+
+    def mean_runs(run):
+        run()
+        rs = [run() for _ in range(max(1, steps))]
+        out = dict(rs[-1])
+        for key in ("time_s", "setup_s", "solve_s"):
+            out[key] = float(np.mean([r[key] for r in rs]))
+        return out
+
+    res = {}
+    common = dict(solver="FCG", rtol=1e-8, prerelax_sweeps=1, postrelax_sweeps=1, dispnorm=0)
+
+    # C0: 7-point 50^3, b = A x*, x* ~ U[-1,1] (seed 0), FP64 BCMG; the CPU oracle then the GPU
+    n, row, col, val = bx.laplacian3d(7, 50, 50, 50)
+    A = bx.CSR(ctx, n, n, row, col, val)
+    xs = np.random.default_rng(0).uniform(-1.0, 1.0, size=n)
+    XS, B, X, Z = (bx.DeviceArray(ctx, (n,)) for _ in range(4))
+    XS.upload(xs)
+    Z.upload(np.zeros(n))
+    A.spmm_device(XS.ptr, B.ptr, 1, n)
+    p = bx.Params(preconditioner="BCMG", precision="FP64", **common)
+    gpu = mean_runs(lambda: timed_solve(bx, ctx, A, B, X, Z, 1, p))
+    gpu["err_vs_xstar"] = float(np.linalg.norm(X.download() - xs) / np.linalg.norm(xs))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+
+    oa = O.laplacian3d(7, 50, 50, 50)
+    b = B.download()
+    t0 = time.perf_counter()
+    op = O.Prec(oa, oracle_params(1e-8))
+    t1 = time.perf_counter()
+    xo, oi = O.fcg(oa, op, oracle_params(1e-8), b)
+    t2 = time.perf_counter()
+    res["c0_lap7_50"] = {"workload": "7-point Laplacian 50^3, b = A x*, x* U[-1,1] seed 0, FCG + FP64 BCMG V(1,1)",
+                         "gpu": gpu, "cpu_oracle": {"time_s": t2 - t0, "setup_s": t1 - t0, "solve_s": t2 - t1,
+                                                    "niter": oi["niter"], "cores": O.lib().or_num_threads(),
+                                                    "err_vs_xstar": float(np.linalg.norm(xo - xs) / np.linalg.norm(xs))}}
+    for h in (XS, B, X, Z, A):
+        h.close()
+
+    # C1: 27-point 128^3, b = A x*, x* ~ U[-1,1] (seed 1), FP64: BCMG and AFSAI
+    n = n128
+    xs = np.random.default_rng(1).uniform(-1.0, 1.0, size=n)
+    XS, B, X, Z = (bx.DeviceArray(ctx, (n,)) for _ in range(4))
+    XS.upload(xs)
+    Z.upload(np.zeros(n))
+    A128.spmm_device(XS.ptr, B.ptr, 1, n)
+    c1 = {"workload": "27-point Laplacian 128^3, b = A x*, x* U[-1,1] seed 1, FCG, all FP64"}
+    for prec in ("BCMG", "AFSAI"):
+        p = bx.Params(preconditioner=prec, precision="FP64", **common)
+        r = mean_runs(lambda: timed_solve(bx, ctx, A128, B, X, Z, 1, p))
+        r["err_vs_xstar"] = float(np.linalg.norm(X.download() - xs) / np.linalg.norm(xs))
+        c1[prec.lower()] = r
+    res["c1_lap27_128"] = c1
+    for h in (XS, B, X, Z):
+        h.close()
+
+    # C2: 7-point diffusion 160^3, conductivities exp(2 g) (seed 2), b ~ U[-1,1] (seed 2):
+    # FP32 hierarchy + V-cycle vs FP64, both under the FP64 FCG
+    n, row, col, val = bx.diffusion7p(160, 160, 160, 0, 2.0, 0.0, 2)
+    A = bx.CSR(ctx, n, n, row, col, val)
+    del row, col, val
+    B, X, Z = (bx.DeviceArray(ctx, (n,)) for _ in range(3))
+    B.upload(np.random.default_rng(2).uniform(-1.0, 1.0, size=n))
+    Z.upload(np.zeros(n))
+    c2 = {"workload": "7-point diffusion 160^3, face conductivity exp(2 g) seed 2, b U[-1,1] seed 2, FCG + BCMG V(1,1)"}
+    sol = {}
+    for precision in ("FP32", "FP64"):
+        p = bx.Params(preconditioner="BCMG", precision=precision, **common)
+        c2[precision.lower()] = mean_runs(lambda: timed_solve(bx, ctx, A, B, X, Z, 1, p))
+        sol[precision] = X.download()
+    c2["fp32_vs_fp64_solution_rel_diff"] = float(np.linalg.norm(sol["FP32"] - sol["FP64"]) / np.linalg.norm(sol["FP64"]))
+    res["c2_diff7_160"] = c2
+    for h in (B, X, Z, A):
+        h.close()
+
+    # C3: the 27-point 128^3 matrix, k = 1, 8, 32, 64 columns N(0,1) (seed 3), FP32 hierarchy, batched
+    c3 = {"workload": "27-point Laplacian 128^3, k columns N(0,1) seed 3, FCG + FP32 BCMG V(1,1), one batched solve"}
+    for k in (1, 8, 32, 64):
+        n = n128
+        B, X, Z = (bx.DeviceArray(ctx, (n, k)) for _ in range(3))
+        B.upload(np.random.default_rng(3).normal(size=(n, k)).ravel(order="F"))
+        Z.upload(np.zeros(n * k))
+        p = bx.Params(preconditioner="BCMG", precision="FP32", **common)
+        r = mean_runs(lambda: timed_solve(bx, ctx, A128, B, X, Z, k, p))
+        r["s_per_column"] = r["time_s"] / k
+        c3[f"k{k}"] = r
+        for h in (B, X, Z):
+            h.close()
+    res["c3_lap27_128_batched"] = c3
+    return res
 
 
 def adjoint_timing(ctx, N=128):
@@ -168,6 +295,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 one-at-a-time comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the autograd forward + adjoint timing")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations' measurements")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
                     help="only time REPS launches of the FP64 SpMV of the fine matrix (profiling)")
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
@@ -277,6 +405,7 @@ def main():
     kernels = kernel_rooflines(bx, ctx, A, n, nnz, k, 20)
 
     adjoint = None if args.no_adjoint else adjoint_timing(ctx)
+    configs = None if args.no_configs else config_measurements(bx, ctx, A, n, 1)
 
     cpu = None if args.no_cpu else cpu_baseline(N, k, rtol)
 
@@ -308,6 +437,7 @@ def main():
         "spmm_fp32_k32_ms": spmm32_ms,
         "stream_copy_gbs": copy_gbs,
         "autograd_forward_adjoint": adjoint,
+        "configs": configs,
         "roofline": {
             "kernel": "autotuned CSR SpMV (k_spmv_stream / k_spmv1), fine 27-point matrix, FP64",
             "bound": "hbm",

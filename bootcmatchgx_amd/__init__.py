@@ -76,6 +76,10 @@ def lib():
         "bcmgx_prec_level_size": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "bcmgx_prec_level_matrix": (C.c_int, [_VP, C.c_int, _I32P, _I32P, _F64P]),
         "bcmgx_prec_level_prolongator": (C.c_int, [_VP, C.c_int, _I32P, _F64P]),
+        "bcmgx_prec_hrc_levels": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int)]),
+        "bcmgx_prec_hrc_level_size": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "bcmgx_prec_hrc_level_matrix": (C.c_int, [_VP, C.c_int, C.c_int, _I32P, _I32P, _F64P]),
+        "bcmgx_prec_hrc_level_prolongator": (C.c_int, [_VP, C.c_int, C.c_int, _I32P, _F64P]),
         "bcmgx_prec_afsai_factor": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), _VP, _VP, _VP]),
         "bcmgx_solve": (C.c_int, [_VP, _VP, _VP, C.POINTER(abi.Params), _VP, _VP, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(abi.SolverOut)]),
@@ -456,29 +460,39 @@ class Prec:
         _check(lib().bcmgx_prec_get_info(self.h, C.byref(i)), self.ctx)
         return {f: getattr(i, f) for f, _ in abi.PrecInfo._fields_}
 
-    def levels(self):
-        return self.info()["num_levels"]
+    def levels(self, h=0):
+        """Levels of bootstrap hierarchy h (boot::H_array[h])."""
+        if h == 0:
+            return self.info()["num_levels"]
+        n = C.c_int()
+        _check(lib().bcmgx_prec_hrc_levels(self.h, h, C.byref(n)), self.ctx)
+        return n.value
 
-    def level_size(self, l):
+    def num_hrc(self):
+        """(boot::n_hrc, boot::estimated_ratio)"""
+        i = self.info()
+        return i["num_hrc"], i["estimated_ratio"]
+
+    def level_size(self, l, h=0):
         n, z = C.c_int(), C.c_int()
-        _check(lib().bcmgx_prec_level_size(self.h, l, C.byref(n), C.byref(z)), self.ctx)
+        _check(lib().bcmgx_prec_hrc_level_size(self.h, h, l, C.byref(n), C.byref(z)), self.ctx)
         return n.value, z.value
 
-    def level_matrix(self, l):
+    def level_matrix(self, l, h=0):
         import scipy.sparse as sp
 
-        n, z = self.level_size(l)
+        n, z = self.level_size(l, h)
         row = np.zeros(n + 1, np.int32)
         col = np.zeros(z, np.int32)
         val = np.zeros(z)
-        _check(lib().bcmgx_prec_level_matrix(self.h, l, row, col, val), self.ctx)
+        _check(lib().bcmgx_prec_hrc_level_matrix(self.h, h, l, row, col, val), self.ctx)
         return sp.csr_matrix((val, col, row), shape=(n, n))
 
-    def level_prolongator(self, l):
-        n, _ = self.level_size(l)
+    def level_prolongator(self, l, h=0):
+        n, _ = self.level_size(l, h)
         agg = np.zeros(n, np.int32)
         pv = np.zeros(n)
-        _check(lib().bcmgx_prec_level_prolongator(self.h, l, agg, pv), self.ctx)
+        _check(lib().bcmgx_prec_hrc_level_prolongator(self.h, h, l, agg, pv), self.ctx)
         return agg, pv
 
     def afsai_factor(self):

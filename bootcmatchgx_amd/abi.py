@@ -76,6 +76,8 @@ class PrecInfo(C.Structure):
         ("max_coarse_size", C.c_int),
         ("setup_seconds", C.c_double),
         ("g_nnz", C.c_longlong),
+        ("num_hrc", C.c_int),
+        ("estimated_ratio", C.c_double),
     ]
 
 

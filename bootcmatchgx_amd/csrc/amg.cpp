@@ -160,7 +160,9 @@ static DBuf<int> check_diagonals(Context* c, const DCsr& A) {
     return dind;
 }
 
-static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H) {
+// w0: the smooth vector the matching weights start from (nullptr: w = 1, as
+// BootBuildData::initByParams fills it, AMG.cu:383-384)
+static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H, const double* w0 = nullptr) {
     BCMGX_CHECK(A0.n == A0.m, BCMGX_ERR_ARG, "the matrix must be square");
     DBuf<int> dind0 = check_diagonals(c, A0);
     int maxcs = p.max_coarse_size > 0 ? p.max_coarse_size : reference_max_coarse_size(A0.n);
@@ -173,7 +175,8 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
     abs_row_sum(c, A0.n, A0.row.get(), A0.val.get(), H.L[0]->M.get());
 
     DBuf<double> w(c, A0.n);
-    fill<double>(c, w.get(), A0.n, 1.0);
+    if (w0) copy_d2d(c, w.get(), w0, sizeof(double) * (size_t)A0.n);
+    else fill<double>(c, w.get(), A0.n, 1.0);
     int ftcoarse = 1;
     double avc = 0.0;
     for (int level = 1; level < p.max_levels;) {
@@ -245,11 +248,11 @@ static void make_fp32_copies(Context* c, Hierarchy& H) {
     for (auto& lp : H.L) {
         Level& l = *lp;
         const DCsr& A = l.mat();
-        DCsr& Am = const_cast<DCsr&>(A);  // the FP32 copy is a cache of the same operator
-        if (!Am.valf) {
-            Am.valf.reset(c, std::max(A.nnz, 1));
-            convert<double, float>(c, A.val.get(), Am.valf.get(), A.nnz);
-        }
+        // the FP32 values live beside the FP64 ones; level 0's (the caller's
+        // matrix) are converted again by every setup, as the coarse levels' are
+        DCsr& Am = const_cast<DCsr&>(A);
+        if (!Am.valf) Am.valf.reset(c, std::max(A.nnz, 1));
+        convert<double, float>(c, A.val.get(), Am.valf.get(), A.nnz);
         l.Mf.reset(c, std::max(A.n, 1));
         convert<double, float>(c, l.M.get(), l.Mf.get(), A.n);
         if (l.pval) {
@@ -309,7 +312,8 @@ void Prec::ensure_work(int k) {
     if (ptype == BCMGX_PREC_BCMG && k > 1 && k != tuned_k) {
         // SpMM column chunk for this batch width (tuned outside any graph capture)
         bool f32 = p.precision == BCMGX_PRECISION_FP32;
-        for (auto& lp : H.L) lp->chunkA = p.autotune ? tune_chunk(ctx, lp->mat(), f32, k) : k;
+        for (int h = 0; h < n_hrc(); h++)
+            for (auto& lp : hrc(h).L) lp->chunkA = p.autotune ? tune_chunk(ctx, lp->mat(), f32, k) : k;
         tuned_k = k;
     }
     if (k <= kcap) return;
@@ -318,27 +322,40 @@ void Prec::ensure_work(int k) {
     // solves hold their addresses, so they are dropped and captured again
     drop_graphs();
     if (ptype == BCMGX_PREC_BCMG) {
-        int nl = H.levels();
+        // the hierarchies of a bootstrap run one after another and share the
+        // cycle buffers: level l is sized for the largest level l among them
+        int nl = 0, ncoarse = 1;
+        for (int h = 0; h < n_hrc(); h++) {
+            nl = std::max(nl, hrc(h).levels());
+            ncoarse = std::max(ncoarse, hrc(h).L.back()->mat().n);
+        }
+        std::vector<size_t> lsz(nl, 1);
+        for (int h = 0; h < n_hrc(); h++)
+            for (int l = 0; l < hrc(h).levels(); l++) lsz[l] = std::max(lsz[l], (size_t)hrc(h).L[l]->mat().n);
         if (p.precision == BCMGX_PRECISION_FP32) {
             wf.clear();
             wf.resize(nl);
             for (int l = 0; l < nl; l++) {
-                size_t sz = (size_t)std::max(H.L[l]->mat().n, 1) * k;
+                size_t sz = lsz[l] * k;
                 wf[l].F.reset(ctx, sz); wf[l].X.reset(ctx, sz); wf[l].X2.reset(ctx, sz); wf[l].Res.reset(ctx, sz);
             }
-            size_t cs = (size_t)std::max(H.L[nl - 1]->mat().n, 1) * k;
+            size_t cs = (size_t)ncoarse * k;
             cd_F.reset(ctx, cs); cd_X.reset(ctx, cs); cd_X2.reset(ctx, cs);
         } else {
             wd.clear();
             wd.resize(nl);
             for (int l = 0; l < nl; l++) {
-                size_t sz = (size_t)std::max(H.L[l]->mat().n, 1) * k;
+                size_t sz = lsz[l] * k;
                 if (l > 0) wd[l].F.reset(ctx, sz);
                 wd[l].X.reset(ctx, sz); wd[l].X2.reset(ctx, sz); wd[l].Res.reset(ctx, sz);
             }
         }
     }
     tmp.reset(ctx, (size_t)std::max(A->n, 1) * k);
+    if (n_hrc() > 1 && p.bootstrap_type == BCMGX_BOOT_ADDITIVE) {
+        tmp2.reset(ctx, (size_t)std::max(A->n, 1) * k);
+        tmp3.reset(ctx, (size_t)std::max(A->n, 1) * k);
+    }
 }
 
 template <typename V>
@@ -389,6 +406,7 @@ template <typename V>
 struct CycleRun {
     Context* c;
     Prec* pr;
+    Hierarchy* H;  // the hierarchy cycled (one of a bootstrap's)
     std::vector<LevelWork<V>>* W;
     int k;
     const int* stop;
@@ -414,7 +432,7 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
     Context* c = R.c;
     Prec* pr = R.pr;
     const bcmgx_params& p = pr->p;
-    Hierarchy& H = pr->H;
+    Hierarchy& H = *R.H;
     int nl = H.levels();
     Level& lv = *H.L[l];
     const DCsr& A = lv.mat();
@@ -499,14 +517,17 @@ void Prec::refresh_level0() {
     if (ptype == BCMGX_PREC_L1_JACOBI) {
         l1_diag(ctx, A->n, A->row.get(), A->col.get(), A->val.get(), pl1j.get());
     } else if (ptype == BCMGX_PREC_BCMG && !H.L.empty()) {
-        Level& l0 = *H.L[0];
-        abs_row_sum(ctx, A->n, A->row.get(), A->val.get(), l0.M.get());
-        if (l0.Mf) convert<double, float>(ctx, l0.M.get(), l0.Mf.get(), A->n);
+        for (int h = 0; h < n_hrc(); h++) {  // every hierarchy's level 0 is A
+            Level& l0 = *hrc(h).L[0];
+            abs_row_sum(ctx, A->n, A->row.get(), A->val.get(), l0.M.get());
+            if (l0.Mf) convert<double, float>(ctx, l0.M.get(), l0.Mf.get(), A->n);
+        }
     }
 }
 
 bool Prec::fused_entry(int k, FcgEntry* e) {
-    if (ptype != BCMGX_PREC_BCMG || p.precision != BCMGX_PRECISION_FP32 || H.levels() < 2 || p.prerelax_sweeps < 1)
+    if (ptype != BCMGX_PREC_BCMG || p.precision != BCMGX_PRECISION_FP32 || H.levels() < 2 || p.prerelax_sweeps < 1 ||
+        n_hrc() > 1)
         return false;
     ensure_work(k);
     e->F = wf[0].F.get();
@@ -516,12 +537,50 @@ bool Prec::fused_entry(int k, FcgEntry* e) {
 }
 
 void Prec::apply_entry_f32(float* z, int k, const int* stop) {
-    BCMGX_CHECK(ptype == BCMGX_PREC_BCMG && p.precision == BCMGX_PRECISION_FP32 && H.levels() >= 2 && p.prerelax_sweeps >= 1,
+    BCMGX_CHECK(ptype == BCMGX_PREC_BCMG && p.precision == BCMGX_PRECISION_FP32 && H.levels() >= 2 && p.prerelax_sweeps >= 1 &&
+                    n_hrc() == 1,
                 BCMGX_ERR_STATE, "level-0 entry not applicable");
     ensure_work(k);
-    CycleRun<float> R{ctx, this, &wf, k, stop};
+    CycleRun<float> R{ctx, this, &H, &wf, k, stop};
     float* res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, nullptr, z);
     if (res != z) copy_d2d(ctx, z, res, sizeof(float) * (size_t)A->n * k);
+}
+
+void Prec::apply_hrc(int h, const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready) {
+    ensure_work(k);
+    Context* c = ctx;
+    long long nk = (long long)A->n * k;
+    Hierarchy& Hh = hrc(h);
+    if (p.precision == BCMGX_PRECISION_FP32) {
+        CycleRun<float> R{c, this, &Hh, &wf, k, stop};
+        float* res;
+        // the last level-0 sweep writes u in FP64 (res == nullptr) when the cycle ends in one
+        if (entry_ready) {
+            BCMGX_CHECK(x_zero && h == 0 && Hh.levels() >= 2 && p.prerelax_sweeps >= 1, BCMGX_ERR_STATE,
+                        "level-0 entry not applicable");
+            res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, u);
+        } else {
+            convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
+            float* xin = nullptr;
+            if (!x_zero) {
+                convert_stop<double, float>(c, u, wf[0].X.get(), nk, stop);
+                xin = wf[0].X.get();
+            }
+            res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin, false, u);
+        }
+        if (res) convert_stop<float, double>(c, res, u, nk, stop);
+    } else {
+        BCMGX_CHECK(!entry_ready, BCMGX_ERR_STATE, "level-0 entry not applicable");
+        CycleRun<double> R{c, this, &Hh, &wd, k, stop};
+        double* xin = nullptr;
+        if (!x_zero) {
+            copy_d2d(c, wd[0].X.get(), u, sizeof(double) * nk);
+            xin = wd[0].X.get();
+        }
+        // the last level-0 sweep writes u directly when the cycle ends in one
+        double* res = gamg_cycle<double>(R, 0, r, xin, false, nullptr, u);
+        if (res != u) convert_stop<double, double>(c, res, u, nk, stop);
+    }
 }
 
 void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready) {
@@ -549,38 +608,86 @@ void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop
         break;
     }
     case BCMGX_PREC_BCMG: {  // bcmg_apply (bcmg.cu:51-90)
-        if (p.precision == BCMGX_PRECISION_FP32) {
-            CycleRun<float> R{c, this, &wf, k, stop};
-            float* res;
-            // the last level-0 sweep writes u in FP64 (res == nullptr) when the cycle ends in one
-            if (entry_ready) {
-                BCMGX_CHECK(x_zero && H.levels() >= 2 && p.prerelax_sweeps >= 1, BCMGX_ERR_STATE, "level-0 entry not applicable");
-                res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, u);
-            } else {
-                convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
-                float* xin = nullptr;
-                if (!x_zero) {
-                    convert_stop<double, float>(c, u, wf[0].X.get(), nk, stop);
-                    xin = wf[0].X.get();
-                }
-                res = gamg_cycle<float>(R, 0, wf[0].F.get(), xin, false, u);
-            }
-            if (res) convert_stop<float, double>(c, res, u, nk, stop);
-        } else {
-            CycleRun<double> R{c, this, &wd, k, stop};
-            double* xin = nullptr;
+        int nh = n_hrc();
+        if (nh == 1) {
+            apply_hrc(0, r, u, k, x_zero, stop, entry_ready);
+            break;
+        }
+        BCMGX_CHECK(!entry_ready, BCMGX_ERR_STATE, "level-0 entry not applicable");
+        if (p.bootstrap_type == BCMGX_BOOT_ADDITIVE) {
+            // u <- u + sum_h B_h (r - A u), the corrections added in hierarchy order
+            const double* res = r;
             if (!x_zero) {
-                copy_d2d(c, wd[0].X.get(), u, sizeof(double) * nk);
-                xin = wd[0].X.get();
+                SpmvArgs a;
+                a.n = A->n; a.row = A->row.get(); a.col = A->col.get(); a.val = A->val.get(); a.nnz = A->nnz;
+                a.x = u; a.y = tmp.get(); a.b = r; a.k = k; a.lanes = cached_lanes(c, *A, false); a.stop = stop;
+                spmv<double, double>(c, SpOp::RES, a);
+                res = tmp.get();
             }
-            // the last level-0 sweep writes u directly when the cycle ends in one
-            double* res = gamg_cycle<double>(R, 0, r, xin, false, nullptr, u);
-            if (res != u) convert_stop<double, double>(c, res, u, nk, stop);
+            // from a zero guess the sum is accumulated in u itself (0 + sum = sum)
+            double* acc = x_zero ? u : tmp2.get();
+            double* t = x_zero ? tmp2.get() : tmp3.get();
+            apply_hrc(0, res, acc, k, true, stop);
+            for (int h = 1; h < nh; h++) {
+                apply_hrc(h, res, t, k, true, stop);
+                add_to(c, acc, t, nk, stop);
+            }
+            if (!x_zero) add_to(c, u, acc, nk, stop);
+        } else {
+            // MULTIPLICATIVE (bcmg.cu:65-85): each hierarchy's cycle starts from the
+            // previous result; SYMMETRIZED_MULTIPLICATIVE adds the way back, h = n-2 .. 0
+            for (int h = 0; h < nh; h++) apply_hrc(h, r, u, k, x_zero && h == 0, stop);
+            if (p.bootstrap_type == BCMGX_BOOT_SYMMETRIZED_MULTIPLICATIVE)
+                for (int h = nh - 2; h >= 0; h--) apply_hrc(h, r, u, k, false, stop);
         }
         break;
     }
     default:
         throw Error(BCMGX_ERR_PARAM, "unhandled preconditioner type");
+    }
+}
+
+// bootstrap (bootstrap.cu:11-46): after the first hierarchy, while fewer than
+// max_hrc exist and the composition of those built reduces the slowest error
+// by no better than conv_ratio per application, one more hierarchy from that
+// error.  The reference leaves the estimate at Boot::init's 1.0 and asserts a
+// single hierarchy (bootstrap.cu:37-38); the estimate here is BootCMatch's:
+// solver_it = 15 (AMG.cu:342) applications of the composition to A x = 0 from a
+// random x, ratio = sqrt(x_15' A x_15 / x_14' A x_14), and the next smooth
+// vector x_15 / max_i |x_15,i|.  The applications are the preconditioner's own
+// (right-hand side 0, guess x); the dot products are fixed-order block sums.
+template <typename Build>
+static void bootstrap(Context* c, const DCsr& A, Prec* pr, Build build_next) {
+    const bcmgx_params& p = pr->p;
+    const int solver_it = 15;
+    int n = A.n;
+    while (pr->n_hrc() < p.max_hrc) {
+        pr->ensure_work(1);
+        DBuf<double> x(c, std::max(n, 1)), ax(c, std::max(n, 1)), zero(c, std::max(n, 1)), dots(c, solver_it + 1);
+        int nblk = fcg_partial_blocks(n);
+        DBuf<double> part(c, std::max(nblk, 1));
+        fill<double>(c, zero.get(), n, 0.0);
+        boot_random(c, x.get(), n, (unsigned long long)pr->n_hrc());
+        auto anorm2 = [&](int slot) {
+            SpmvArgs a;
+            a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+            a.x = x.get(); a.y = ax.get(); a.lanes = cached_lanes(c, A, false);
+            spmv<double, double>(c, SpOp::AX, a);
+            block_dots(c, n, 1, x.get(), ax.get(), 0, part.get(), nblk, dots.get() + slot);
+        };
+        anorm2(0);
+        for (int it = 1; it <= solver_it; it++) {
+            pr->apply(zero.get(), x.get(), 1, false, nullptr);  // x <- (I - B A) x
+            anorm2(it);
+        }
+        double a[2];
+        c->d2h(a, dots.get() + solver_it - 1, 2);
+        pr->boot_ratio = a[0] > 0.0 ? std::sqrt(a[1] / a[0]) : 0.0;
+        if (!(pr->boot_ratio > p.conv_ratio)) break;
+        scale_by_absmax(c, x.get(), n);
+        build_next(x.get());
+        pr->kcap = 0;  // the cycle buffers are sized again for the new hierarchy
+        pr->tuned_k = 0;
     }
 }
 
@@ -604,20 +711,28 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
         pr->lanesGT = default_lanes(pr->GT.nnz, pr->GT.n);
         break;
     case BCMGX_PREC_BCMG: {
-        BCMGX_CHECK(p.max_hrc == 1, BCMGX_ERR_UNSUPPORTED, "bootstrap with more than one hierarchy is not supported (bootstrap.cu:37-38)");
         BCMGX_CHECK(p.aggr_type == 0, BCMGX_ERR_UNSUPPORTED, "aggr_type != 0 is not supported");
-        BCMGX_CHECK(p.bootstrap_type == BCMGX_BOOT_MULTIPLICATIVE, BCMGX_ERR_UNSUPPORTED,
-                    "only the MULTIPLICATIVE composition is applied (bcmg.cu:60-87)");
+        BCMGX_CHECK(p.bootstrap_type >= 0 && p.bootstrap_type < BCMGX_BOOT_INVALID, BCMGX_ERR_PARAM,
+                    "invalid bootstrap_type");
+        BCMGX_CHECK(p.max_hrc >= 1, BCMGX_ERR_PARAM, "max_hrc must be at least 1");
         BCMGX_CHECK(p.cycle_type >= 0 && p.cycle_type < BCMGX_CYCLE_INVALID, BCMGX_ERR_PARAM, "invalid cycle_type");
-        build_hierarchy(c, A, p, pr->H);
         bool f32 = p.precision == BCMGX_PRECISION_FP32;
-        if (f32) make_fp32_copies(c, pr->H);
-        for (auto& lp : pr->H.L) {
-            Level& l = *lp;
-            const DCsr& M = l.mat();
-            l.lanesA = p.autotune ? tune_lanes(c, M, f32) : default_lanes(M.nnz, M.n);
-            l.lanesR = l.R.n > 0 ? default_lanes(l.R.nnz, l.R.n) : 1;
-        }
+        auto finish = [&](Hierarchy& H) {
+            if (f32) make_fp32_copies(c, H);
+            for (auto& lp : H.L) {
+                Level& l = *lp;
+                const DCsr& M = l.mat();
+                l.lanesA = p.autotune ? tune_lanes(c, M, f32) : default_lanes(M.nnz, M.n);
+                l.lanesR = l.R.n > 0 ? default_lanes(l.R.nnz, l.R.n) : 1;
+            }
+        };
+        build_hierarchy(c, A, p, pr->H);
+        finish(pr->H);
+        bootstrap(c, A, pr.get(), [&](const double* w) {
+            pr->HB.emplace_back(new Hierarchy());
+            build_hierarchy(c, A, p, *pr->HB.back(), w);
+            finish(*pr->HB.back());
+        });
         break;
     }
     default:

@@ -43,7 +43,11 @@ struct Prec {
     const DCsr* A = nullptr;
     bcmgx_params p{};
     int ptype = BCMGX_PREC_NONE;
-    Hierarchy H;
+    Hierarchy H;                                   // the first hierarchy (boot->H_array[0])
+    std::vector<std::unique_ptr<Hierarchy>> HB;    // bootstrap hierarchies 1 .. n_hrc-1
+    double boot_ratio = 1.0;                       // boot->estimated_ratio (1 until estimated)
+    int n_hrc() const { return 1 + (int)HB.size(); }
+    Hierarchy& hrc(int h) { return h == 0 ? H : *HB[h - 1]; }
     // one-level l1-Jacobi
     DBuf<double> pl1j;
     // AFSAI
@@ -56,6 +60,7 @@ struct Prec {
     std::vector<LevelWork<float>> wf;
     DBuf<double> cd_F, cd_X, cd_X2;  // FP64 coarsest-level buffers of the FP32 cycle
     DBuf<double> tmp;                // AFSAI intermediate / prec_apply staging
+    DBuf<double> tmp2, tmp3;         // additive bootstrap composition: the sum and one hierarchy's correction
     double setup_seconds = 0;
     // solve workspaces per k (graphs are bound to their buffers)
     std::vector<std::unique_ptr<SolveWork>> solves;
@@ -70,6 +75,8 @@ struct Prec {
     // `entry_ready`: the level-0 entry of the FP32 cycle (see fused_entry) already
     // holds F = (float) r and the first sweep X = F / M.
     void apply(const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready = false);
+    // one cycle of bootstrap hierarchy h alone: u <- cycle_h(r) from the guess u
+    void apply_hrc(int h, const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready = false);
     // the level-0 buffers an FCG update can fill for the next apply (FP32 BCMG
     // with at least two levels and a pre-smoothing sweep); false otherwise
     bool fused_entry(int k, FcgEntry* e);

@@ -352,27 +352,51 @@ int bcmgx_prec_get_info(bcmgx_prec prec, bcmgx_prec_info* info) {
     info->max_coarse_size = P->H.maxcs;
     info->setup_seconds = P->setup_seconds;
     info->g_nnz = P->G.nnz;
+    info->num_hrc = P->ptype == BCMGX_PREC_BCMG ? P->n_hrc() : 0;
+    info->estimated_ratio = P->boot_ratio;
     return BCMGX_OK;
 }
 
-int bcmgx_prec_level_size(bcmgx_prec prec, int level, int* n, int* nnz) {
+int bcmgx_prec_hrc_levels(bcmgx_prec prec, int h, int* num_levels) {
+    if (!prec || !num_levels) return BCMGX_ERR_ARG;
+    if (!prec->ctx) return BCMGX_ERR_STATE;
+    Prec* P = prec->P.get();
+    if (P->ptype != BCMGX_PREC_BCMG || h < 0 || h >= P->n_hrc()) return BCMGX_ERR_ARG;
+    *num_levels = P->hrc(h).levels();
+    return BCMGX_OK;
+}
+
+int bcmgx_prec_hrc_level_size(bcmgx_prec prec, int h, int level, int* n, int* nnz) {
     if (!prec || level < 0) return BCMGX_ERR_ARG;
     if (!prec->ctx) return BCMGX_ERR_STATE;
     Prec* P = prec->P.get();
-    if (P->ptype != BCMGX_PREC_BCMG || level >= P->H.levels()) return BCMGX_ERR_ARG;
-    const DCsr& A = P->H.L[level]->mat();
+    if (P->ptype != BCMGX_PREC_BCMG || h < 0 || h >= P->n_hrc() || level >= P->hrc(h).levels()) return BCMGX_ERR_ARG;
+    const DCsr& A = P->hrc(h).L[level]->mat();
     if (n) *n = A.n;
     if (nnz) *nnz = A.nnz;
     return BCMGX_OK;
 }
 
+int bcmgx_prec_level_size(bcmgx_prec prec, int level, int* n, int* nnz) {
+    return bcmgx_prec_hrc_level_size(prec, 0, level, n, nnz);
+}
+
 int bcmgx_prec_level_matrix(bcmgx_prec prec, int level, int* row, int* col, double* val) {
+    return bcmgx_prec_hrc_level_matrix(prec, 0, level, row, col, val);
+}
+
+int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* pval) {
+    return bcmgx_prec_hrc_level_prolongator(prec, 0, level, agg, pval);
+}
+
+int bcmgx_prec_hrc_level_matrix(bcmgx_prec prec, int h, int level, int* row, int* col, double* val) {
     if (!prec || level < 0) return BCMGX_ERR_ARG;
     if (!prec->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Prec* P = prec->P.get();
-    BCMGX_CHECK(P->ptype == BCMGX_PREC_BCMG && level < P->H.levels(), BCMGX_ERR_ARG, "no such level");
-    const DCsr& A = P->H.L[level]->mat();
+    BCMGX_CHECK(P->ptype == BCMGX_PREC_BCMG && h >= 0 && h < P->n_hrc() && level < P->hrc(h).levels(), BCMGX_ERR_ARG,
+                "no such level");
+    const DCsr& A = P->hrc(h).L[level]->mat();
     Context* c = prec->ctx->c;
     if (row) c->d2h(row, A.row.get(), (size_t)A.n + 1);
     if (col && A.nnz) c->d2h(col, A.col.get(), A.nnz);
@@ -381,13 +405,14 @@ int bcmgx_prec_level_matrix(bcmgx_prec prec, int level, int* row, int* col, doub
     API_END(prec->ctx)
 }
 
-int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* pval) {
+int bcmgx_prec_hrc_level_prolongator(bcmgx_prec prec, int h, int level, int* agg, double* pval) {
     if (!prec || level < 0) return BCMGX_ERR_ARG;
     if (!prec->ctx) return BCMGX_ERR_STATE;
     API_BEGIN
     Prec* P = prec->P.get();
-    BCMGX_CHECK(P->ptype == BCMGX_PREC_BCMG && level < P->H.levels() - 1, BCMGX_ERR_ARG, "no prolongator at this level");
-    Level& l = *P->H.L[level];
+    BCMGX_CHECK(P->ptype == BCMGX_PREC_BCMG && h >= 0 && h < P->n_hrc() && level < P->hrc(h).levels() - 1, BCMGX_ERR_ARG,
+                "no prolongator at this level");
+    Level& l = *P->hrc(h).L[level];
     Context* c = prec->ctx->c;
     int n = l.mat().n;
     if (agg) c->d2h(agg, l.agg.get(), n);
@@ -506,8 +531,8 @@ int bcmgx_dump_info(const char* path, const bcmgx_params* p, bcmgx_prec prec, co
     std::fprintf(fp, "Final residual                               : %.10lf\n", out->exit_res);
     if (p->preconditioner == BCMGX_PREC_BCMG && prec && prec->P) {
         Prec* P = prec->P.get();
-        std::fprintf(fp, "Number of hierarchies                        : %d\n", 1);
-        std::fprintf(fp, "Estimated ratio                              : %f\n", 1.0);
+        std::fprintf(fp, "Number of hierarchies                        : %d\n", P->n_hrc());
+        std::fprintf(fp, "Estimated ratio                              : %f\n", P->boot_ratio);
         std::fprintf(fp, "max_hrc                                      : %d\n", p->max_hrc);
         std::fprintf(fp, "conv_ratio                                   : %f\n", p->conv_ratio);
         std::fprintf(fp, "bootstrap_composition_type                   : %s\n", bcmgx_enum_to_string("bootstrap_type", p->bootstrap_type));

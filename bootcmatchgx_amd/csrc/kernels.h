@@ -29,6 +29,13 @@ void fill(Context* c, T* p, long long n, T v);
 template <typename A, typename B>
 void convert(Context* c, const A* s, B* d, long long n);
 void copy_d2d(Context* c, void* dst, const void* src, size_t bytes);
+// bootstrap test vectors: x_i = 2 u_i - 1, u_i the top 53 bits of
+// splitmix64(seed + (i + 1) * 0x9E3779B97F4A7C15) scaled to [0, 1)
+void boot_random(Context* c, double* x, long long n, unsigned long long seed);
+// x_i <- x_i / max_j |x_j| (the maximum is order-independent; x unchanged when it is 0)
+void scale_by_absmax(Context* c, double* x, long long n);
+// y_i <- y_i + x_i
+void add_to(Context* c, double* y, const double* x, long long n, const int* stop);
 double stream_copy_gbs(Context* c, size_t bytes, int reps);
 
 // ---- sparse matrix-vector family (spmv.hip) ---------------------------------

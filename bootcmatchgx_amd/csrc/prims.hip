@@ -249,6 +249,67 @@ template void convert<double, float>(Context*, const double*, float*, long long)
 template void convert<float, double>(Context*, const float*, double*, long long);
 template void convert<double, double>(Context*, const double*, double*, long long);
 
+__global__ __launch_bounds__(WG) void k_boot_random(double* __restrict__ x, long long n, unsigned long long seed) {
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < n; i += (long long)gridDim.x * WG) {
+        unsigned long long z = seed + (unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        double u01 = (double)(z >> 11) * 0x1.0p-53;
+        x[i] = 2.0 * u01 - 1.0;
+    }
+}
+
+void boot_random(Context* c, double* x, long long n, unsigned long long seed) {
+    if (n <= 0) return;
+    k_boot_random<<<grid_for(n, WG), WG, 0, c->stream>>>(x, n, seed);
+    BCMGX_LAUNCHED();
+}
+
+// block maxima of |x|, then one block takes their maximum and scales x by it in
+// a second launch (max is exact in any order, so the result is the oracle's)
+__global__ __launch_bounds__(WG) void k_absmax_partial(const double* __restrict__ x, long long n, double* __restrict__ part) {
+    __shared__ double s[WG / 64];
+    double m = 0.0;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < n; i += (long long)gridDim.x * WG) m = fmax(m, fabs(x[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < WG / 64; w++) m = fmax(m, s[w]);
+        part[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_scale_by_max(double* __restrict__ x, long long n, const double* __restrict__ part, int nb) {
+    double m = 0.0;
+    for (int b = 0; b < nb; b++) m = fmax(m, part[b]);
+    if (!(m > 0.0)) return;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < n; i += (long long)gridDim.x * WG) x[i] = x[i] / m;
+}
+
+void scale_by_absmax(Context* c, double* x, long long n) {
+    if (n <= 0) return;
+    int nb = (int)std::min<long long>(1024, (n + WG - 1) / WG);
+    DBuf<double> part(c, nb);
+    k_absmax_partial<<<nb, WG, 0, c->stream>>>(x, n, part.get());
+    BCMGX_LAUNCHED();
+    k_scale_by_max<<<nb, WG, 0, c->stream>>>(x, n, part.get(), nb);
+    BCMGX_LAUNCHED();
+}
+
+__global__ __launch_bounds__(WG) void k_add_to(double* __restrict__ y, const double* __restrict__ x, long long n,
+                                               const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < n; i += (long long)gridDim.x * WG) y[i] = y[i] + x[i];
+}
+
+void add_to(Context* c, double* y, const double* x, long long n, const int* stop) {
+    if (n <= 0) return;
+    k_add_to<<<grid_for(n, WG), WG, 0, c->stream>>>(y, x, n, stop);
+    BCMGX_LAUNCHED();
+}
+
 void copy_d2d(Context* c, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
     BCMGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));

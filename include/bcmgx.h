@@ -168,6 +168,16 @@ int bcmgx_memcpy(bcmgx_context ctx, void* dst, const void* src, size_t bytes, in
 /* ---- preconditioner -----------------------------------------------------
  * prec_setup (src/preconditioner/prec_setup.h:55): the preconditioner type
  * comes from params->preconditioner (the reference's cgsprec::ptype).
+ * BCMG with max_hrc > 1 is the bootstrap (bootstrap.cu:11-46): after the
+ * first hierarchy, while fewer than max_hrc exist and the composition of those
+ * built so far has an estimated convergence ratio above conv_ratio, one more
+ * hierarchy is built from the slowest error of that composition (15
+ * applications to A x = 0 from a random x, whose last A-norm ratio is the
+ * estimate).  bootstrap_type composes them in prec_apply: MULTIPLICATIVE
+ * (bcmg.cu:65-85), SYMMETRIZED_MULTIPLICATIVE (forward then back, a symmetric
+ * preconditioner) or ADDITIVE (corrections summed from one residual).  The
+ * reference stops at one hierarchy (bootstrap.cu:37-38) and applies
+ * MULTIPLICATIVE only (bcmg.cu:86-89).
  */
 int bcmgx_prec_setup(bcmgx_context ctx, bcmgx_matrix A, const bcmgx_params* p, bcmgx_prec* prec);
 /* prec_apply (src/preconditioner/prec_apply.h:32): u <- B r.  As in the
@@ -186,6 +196,8 @@ typedef struct bcmgx_prec_info {
     int max_coarse_size;    /* limit used (reference formula or extension key) */
     double setup_seconds;   /* device time of the setup (hipEvents) */
     long long g_nnz;        /* AFSAI: nnz of G */
+    int num_hrc;            /* BCMG: hierarchies built by the bootstrap (boot::n_hrc, AMG.h:91) */
+    double estimated_ratio; /* BCMG: boot::estimated_ratio (AMG.h:92), 1 until estimated */
 } bcmgx_prec_info;
 int bcmgx_prec_get_info(bcmgx_prec prec, bcmgx_prec_info* info);
 /* level sizes: n and nnz of A_level */
@@ -194,6 +206,12 @@ int bcmgx_prec_level_size(bcmgx_prec prec, int level, int* n, int* nnz);
 int bcmgx_prec_level_matrix(bcmgx_prec prec, int level, int* row, int* col, double* val);
 /* prolongator of level -> level+1: one entry per fine row: coarse index and value */
 int bcmgx_prec_level_prolongator(bcmgx_prec prec, int level, int* agg, double* pval);
+/* The same for hierarchy h of a bootstrap (boot::H_array[h], AMG.h:88-93);
+ * h = 0 is the hierarchy the three calls above read. */
+int bcmgx_prec_hrc_levels(bcmgx_prec prec, int h, int* num_levels);
+int bcmgx_prec_hrc_level_size(bcmgx_prec prec, int h, int level, int* n, int* nnz);
+int bcmgx_prec_hrc_level_matrix(bcmgx_prec prec, int h, int level, int* row, int* col, double* val);
+int bcmgx_prec_hrc_level_prolongator(bcmgx_prec prec, int h, int level, int* agg, double* pval);
 /* AFSAI factor G (lower triangular, row-sorted) */
 int bcmgx_prec_afsai_factor(bcmgx_prec prec, int* n, int* nnz, int* row, int* col, double* val);
 

@@ -557,13 +557,15 @@ Csr matching_aggregation(const Csr& A, std::vector<double>& w, int sweeps, int m
 }
 
 // adaptiveCoarsening (matchingAggregation.cu:287-466) + finalize_cmplx/wcmplx (AMG.cu:143-163)
-void build_hierarchy(const Csr& A0, const bcmgx_params& p, Hierarchy& H)
+// w0: the smooth vector the matching weights start from (BootBuildData::initByParams
+// fills it with 1, AMG.cu:383-384; a bootstrap hierarchy starts from its own)
+void build_hierarchy(const Csr& A0, const bcmgx_params& p, Hierarchy& H, const std::vector<double>* w0 = nullptr)
 {
     int maxcs = p.max_coarse_size > 0 ? p.max_coarse_size : max_coarse_size(A0.n);
     H.maxcoarse = maxcs;
     int sweeps = p.aggrsweeps + 1;
     int maxlevels = p.max_levels;
-    std::vector<double> w(A0.n, 1.0);
+    std::vector<double> w = w0 ? *w0 : std::vector<double>(A0.n, 1.0);
     H.L.clear();
     H.L.push_back(Level());
     H.L[0].A = A0;
@@ -848,12 +850,102 @@ struct ocsr { Csr A; };
 struct oprec {
     bcmgx_params p;
     const Csr* A = nullptr;
-    Hierarchy H;
+    Hierarchy H;                 // the first hierarchy (boot->H_array[0], bcmg.cu:31)
+    std::vector<Hierarchy> HB;   // bootstrap hierarchies 1 .. n_hrc-1
+    double est_ratio = 1.0;      // boot->estimated_ratio (Boot::init(max_hrc, 1.0), AMG.cu:393-399)
     Afsai af;
     std::vector<double> pl1j;
-    mutable CycleWork<double> Wd;
-    mutable CycleWork<float> Wf;
+    mutable std::vector<CycleWork<double>> Wd;  // one per hierarchy
+    mutable std::vector<CycleWork<float>> Wf;
+    int n_hrc() const { return 1 + (int)HB.size(); }
+    const Hierarchy& hrc(int h) const { return h == 0 ? H : HB[h - 1]; }
 };
+
+namespace {
+
+// One V-cycle (or W/H/variable-V) of hierarchy h: u <- cycle(r) from the
+// initial guess u (prec_apply's in/out argument, bcmg.cu:70-81).
+void cycle_h(const oprec* pr, int h, const double* r, double* u)
+{
+    const Hierarchy& H = pr->hrc(h);
+    const bcmgx_params& p = pr->p;
+    int n = pr->A->n;
+    if (p.precision == BCMGX_PRECISION_FP32) {
+        auto& W = pr->Wf[h];
+        std::vector<const std::vector<float>*> Av, Mv, Rv, Pv;
+        for (auto& l : H.L) { Av.push_back(&l.Af); Mv.push_back(&l.Mf); Rv.push_back(&l.Rf); Pv.push_back(&l.Pf); }
+        for (int i = 0; i < n; i++) { W.F[0][i] = (float)r[i]; W.X[0][i] = (float)u[i]; }
+        // the coarsest level is smoothed in FP64 (see the HIP path)
+        gamg_cycle<float>(H, p, W, 0, Av, Mv, Rv, Pv);
+        for (int i = 0; i < n; i++) u[i] = (double)W.X[0][i];
+    } else {
+        auto& W = pr->Wd[h];
+        std::vector<const std::vector<double>*> Av, Mv, Rv, Pv;
+        for (auto& l : H.L) { Av.push_back(&l.A.val); Mv.push_back(&l.M); Rv.push_back(&l.R.val); Pv.push_back(&l.pval); }
+        std::copy(r, r + n, W.F[0].begin());
+        std::copy(u, u + n, W.X[0].begin());
+        gamg_cycle<double>(H, p, W, 0, Av, Mv, Rv, Pv);
+        std::copy(W.X[0].begin(), W.X[0].end(), u);
+    }
+}
+
+// bcmg_apply (bcmg.cu:51-90) over the n_hrc hierarchies of a bootstrap.
+//  MULTIPLICATIVE (bcmg.cu:65-85): u <- cycle_h(r, guess u) for h = 0 .. n-1,
+//    the error propagator E_{n-1} ... E_0 (E_h = I - B_h A).
+//  SYMMETRIZED_MULTIPLICATIVE: h = 0 .. n-1, then h = n-2 .. 0 (the last
+//    hierarchy once): E_0 ... E_{n-1} ... E_0, A-self-adjoint when every E_h is,
+//    so the preconditioner is symmetric.
+//  ADDITIVE: u <- u + sum_h B_h (r - A u), summed in hierarchy order.
+// The reference applies MULTIPLICATIVE only and exits on the other two
+// (bcmg.cu:86-89); the last two are the compositions BootCMatch defines.
+// With one hierarchy all three are the single cycle.
+void bcmg_compose(const oprec* pr, const double* r, double* u)
+{
+    int nh = pr->n_hrc();
+    int n = pr->A->n;
+    if (nh == 1) {
+        cycle_h(pr, 0, r, u);
+        return;
+    }
+    switch (pr->p.bootstrap_type) {
+    case BCMGX_BOOT_MULTIPLICATIVE:
+        for (int h = 0; h < nh; h++) cycle_h(pr, h, r, u);
+        break;
+    case BCMGX_BOOT_SYMMETRIZED_MULTIPLICATIVE:
+        for (int h = 0; h < nh; h++) cycle_h(pr, h, r, u);
+        for (int h = nh - 2; h >= 0; h--) cycle_h(pr, h, r, u);
+        break;
+    default: {  // ADDITIVE
+        std::vector<double> res(n), t(n);
+        spmv(*pr->A, u, t.data(), 1.0, 0.0, 1);
+        for (int i = 0; i < n; i++) res[i] = r[i] - t[i];
+        std::vector<double> acc(n, 0.0);
+        for (int h = 0; h < nh; h++) {
+            std::fill(t.begin(), t.end(), 0.0);
+            cycle_h(pr, h, res.data(), t.data());
+            for (int i = 0; i < n; i++) acc[i] += t[i];
+        }
+        for (int i = 0; i < n; i++) u[i] += acc[i];
+        break;
+    }
+    }
+}
+
+}  // namespace
+
+// Random start of the bootstrap's test vectors: x_i = 2 u_i - 1, u_i the top 53
+// bits of splitmix64(seed + (i + 1) * golden gamma) scaled to [0, 1).
+void or_boot_random(int n, unsigned long long seed, double* x)
+{
+    for (int i = 0; i < n; i++) {
+        unsigned long long z = seed + (unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        double u01 = (double)(z >> 11) * 0x1.0p-53;
+        x[i] = 2.0 * u01 - 1.0;
+    }
+}
 
 static void prec_apply_impl(const oprec* pr, const double* r, double* u)
 {
@@ -875,29 +967,9 @@ static void prec_apply_impl(const oprec* pr, const double* r, double* u)
         spmv(pr->af.GT, t.data(), u, 1.0, 0.0, 1);
         break;
     }
-    case BCMGX_PREC_BCMG: {  // bcmg_apply (bcmg.cu:51-90)
-        const Hierarchy& H = pr->H;
-        int nl = (int)H.L.size();
-        if (p.precision == BCMGX_PRECISION_FP32) {
-            auto& W = pr->Wf;
-            std::vector<const std::vector<float>*> Av, Mv, Rv, Pv;
-            for (auto& l : H.L) { Av.push_back(&l.Af); Mv.push_back(&l.Mf); Rv.push_back(&l.Rf); Pv.push_back(&l.Pf); }
-            for (int i = 0; i < n; i++) { W.F[0][i] = (float)r[i]; W.X[0][i] = (float)u[i]; }
-            // the coarsest level is smoothed in FP64 (see the HIP path)
-            gamg_cycle<float>(H, p, W, 0, Av, Mv, Rv, Pv);
-            for (int i = 0; i < n; i++) u[i] = (double)W.X[0][i];
-            (void)nl;
-        } else {
-            auto& W = pr->Wd;
-            std::vector<const std::vector<double>*> Av, Mv, Rv, Pv;
-            for (auto& l : H.L) { Av.push_back(&l.A.val); Mv.push_back(&l.M); Rv.push_back(&l.R.val); Pv.push_back(&l.pval); }
-            std::copy(r, r + n, W.F[0].begin());
-            std::copy(u, u + n, W.X[0].begin());
-            gamg_cycle<double>(H, p, W, 0, Av, Mv, Rv, Pv);
-            std::copy(W.X[0].begin(), W.X[0].end(), u);
-        }
+    case BCMGX_PREC_BCMG:  // bcmg_apply (bcmg.cu:51-90)
+        bcmg_compose(pr, r, u);
         break;
-    }
     default:
         break;
     }
@@ -993,15 +1065,54 @@ oprec* or_prec_setup(const ocsr* A, const bcmgx_params* p)
         afsai_setup(A->A, pr->af);
         break;
     case BCMGX_PREC_BCMG: {
-        build_hierarchy(A->A, *p, pr->H);
-        auto alloc = [&](auto& W) {
-            for (auto& l : pr->H.L) {
-                int n = l.A.n;
-                W.F.emplace_back(n); W.X.emplace_back(n); W.X2.emplace_back(n); W.Res.emplace_back(n);
-            }
+        // bootstrap (bootstrap.cu:11-46): the first hierarchy from w = 1; while
+        // fewer than max_hrc exist and the composition of those built so far
+        // reduces the slowest error by no better than conv_ratio per application,
+        // one more hierarchy from that error.  The reference sets the estimate to
+        // 1 and never updates it (it asserts a single hierarchy, bootstrap.cu:37-38);
+        // the estimate here is BootCMatch's: solver_it (15, AMG.cu:342) applications
+        // of the composition to A x = 0 from a random x, the ratio the A-norm of
+        // the last one, sqrt(x_it' A x_it / x_{it-1}' A x_{it-1}), and the next
+        // hierarchy's smooth vector x_it / max_i |x_it,i|.
+        const Csr& A0 = A->A;
+        int n = A0.n;
+        auto alloc = [&](const Hierarchy& H) {
+            auto add = [&](auto& Ws) {
+                Ws.emplace_back();
+                auto& W = Ws.back();
+                for (auto& l : H.L) {
+                    int m = l.A.n;
+                    W.F.emplace_back(m); W.X.emplace_back(m); W.X2.emplace_back(m); W.Res.emplace_back(m);
+                }
+            };
+            if (p->precision == BCMGX_PRECISION_FP32) add(pr->Wf);
+            else add(pr->Wd);
         };
-        if (p->precision == BCMGX_PRECISION_FP32) alloc(pr->Wf);
-        else alloc(pr->Wd);
+        build_hierarchy(A0, *p, pr->H);
+        alloc(pr->H);
+        const int solver_it = 15;  // bootBuildData::solver_it (AMG.cu:342)
+        while (pr->n_hrc() < p->max_hrc) {
+            std::vector<double> x(n), ax(n), zero(n, 0.0);
+            or_boot_random(n, (unsigned long long)pr->n_hrc(), x.data());
+            spmv(A0, x.data(), ax.data(), 1.0, 0.0, 1);
+            double a_cur = dot(x.data(), ax.data(), n), a_prev = 0.0;
+            for (int it = 0; it < solver_it; it++) {
+                bcmg_compose(pr, zero.data(), x.data());  // x <- (I - B A) x
+                spmv(A0, x.data(), ax.data(), 1.0, 0.0, 1);
+                a_prev = a_cur;
+                a_cur = dot(x.data(), ax.data(), n);
+            }
+            pr->est_ratio = a_prev > 0.0 ? std::sqrt(a_cur / a_prev) : 0.0;
+            if (!(pr->est_ratio > p->conv_ratio)) break;
+            double m = 0.0;
+            for (int i = 0; i < n; i++) m = std::max(m, std::fabs(x[i]));
+            if (!(m > 0.0)) break;
+            std::vector<double> w(n);
+            for (int i = 0; i < n; i++) w[i] = x[i] / m;
+            pr->HB.emplace_back();
+            build_hierarchy(A0, *p, pr->HB.back(), &w);
+            alloc(pr->HB.back());
+        }
         break;
     }
     default:
@@ -1013,6 +1124,46 @@ oprec* or_prec_setup(const ocsr* A, const bcmgx_params* p)
 void or_prec_free(oprec* pr) { delete pr; }
 int or_prec_levels(const oprec* pr) { return (int)pr->H.L.size(); }
 const ocsr* or_prec_level_A(const oprec* pr, int level) { return reinterpret_cast<const ocsr*>(&pr->H.L[level].A); }
+int or_prec_num_hrc(const oprec* pr, double* estimated_ratio)
+{
+    if (estimated_ratio) *estimated_ratio = pr->est_ratio;
+    return pr->n_hrc();
+}
+int or_prec_hrc_levels(const oprec* pr, int h) { return (int)pr->hrc(h).L.size(); }
+void or_prec_apply_hrc(const oprec* pr, int h, const double* r, double* u) { cycle_h(pr, h, r, u); }
+oprec* or_prec_setup_w(const ocsr* A, const bcmgx_params* p, const double* w)
+{
+    if (p->preconditioner != BCMGX_PREC_BCMG) return nullptr;
+    bcmgx_params q = *p;
+    q.max_hrc = 1;
+    oprec* pr = or_prec_setup(A, &q);
+    std::vector<double> wv(w, w + A->A.n);
+    pr->H = Hierarchy();
+    build_hierarchy(A->A, q, pr->H, &wv);
+    pr->Wd.clear();
+    pr->Wf.clear();
+    auto add = [&](auto& Ws) {
+        Ws.emplace_back();
+        for (auto& l : pr->H.L) {
+            int m = l.A.n;
+            Ws.back().F.emplace_back(m); Ws.back().X.emplace_back(m); Ws.back().X2.emplace_back(m); Ws.back().Res.emplace_back(m);
+        }
+    };
+    if (q.precision == BCMGX_PRECISION_FP32) add(pr->Wf);
+    else add(pr->Wd);
+    return pr;
+}
+const ocsr* or_prec_hrc_level_A(const oprec* pr, int h, int level)
+{
+    return reinterpret_cast<const ocsr*>(&pr->hrc(h).L[level].A);
+}
+int or_prec_hrc_level_P(const oprec* pr, int h, int level, int* agg, double* pval)
+{
+    const Level& l = pr->hrc(h).L[level];
+    std::copy(l.agg.begin(), l.agg.end(), agg);
+    std::copy(l.pval.begin(), l.pval.end(), pval);
+    return (int)l.agg.size();
+}
 int or_prec_level_P(const oprec* pr, int level, int* agg, double* pval)
 {
     const Level& l = pr->H.L[level];

@@ -61,6 +61,18 @@ int or_prec_level_P(const oprec* pr, int level, int* agg, double* pval);
 void or_prec_hierarchy_stats(const oprec* pr, double* op_cmplx, double* op_wcmplx, double* avg_cratio, int* maxcoarse);
 const ocsr* or_prec_afsai_G(const oprec* pr);
 void or_prec_apply(const oprec* pr, const double* r, double* u);
+/* bootstrap: number of hierarchies and the last estimated convergence ratio
+ * (1 when none was estimated); hierarchy h's levels, matrices, prolongators */
+int or_prec_num_hrc(const oprec* pr, double* estimated_ratio);
+int or_prec_hrc_levels(const oprec* pr, int h);
+const ocsr* or_prec_hrc_level_A(const oprec* pr, int h, int level);
+int or_prec_hrc_level_P(const oprec* pr, int h, int level, int* agg, double* pval);
+/* one cycle of hierarchy h alone (u in/out, the initial guess) */
+void or_prec_apply_hrc(const oprec* pr, int h, const double* r, double* u);
+/* a one-hierarchy BCMG preconditioner whose matching starts from the smooth vector w */
+oprec* or_prec_setup_w(const ocsr* A, const bcmgx_params* p, const double* w);
+/* the bootstrap's random start vectors: x_i = 2 u_i - 1, u_i from splitmix64 */
+void or_boot_random(int n, unsigned long long seed, double* x);
 
 /* FCG (FCG.cu:27-175).  x in/out.  Returns retv. */
 int or_fcg(const ocsr* A, const oprec* pr, const bcmgx_params* p, const double* b, double* x,

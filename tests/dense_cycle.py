@@ -28,12 +28,12 @@ class DenseHierarchy:
         self.M = [np.abs(a).sum(axis=1) for a in self.A]
 
     @classmethod
-    def from_oracle(cls, A, P):
-        """The oracle's prolongators; everything else recomputed here."""
+    def from_oracle(cls, A, P, h=0):
+        """The oracle's prolongators (of bootstrap hierarchy h); everything else recomputed here."""
         pro = []
-        for l in range(P.levels() - 1):
-            agg, pv = P.level_P(l)
-            pro.append((agg, pv, P.level_A(l + 1).dims()[0]))
+        for l in range(P.hrc_levels(h) - 1):
+            agg, pv = P.hrc_level_P(h, l)
+            pro.append((agg, pv, P.hrc_level_A(h, l + 1).dims()[0]))
         return cls(A.to_scipy().toarray(), pro)
 
     def levels(self):
@@ -46,9 +46,11 @@ def np_cycle(H, p, l, f, x0=None):
     A, M = H.A[l], H.M[l]
     x = np.zeros_like(f) if x0 is None else x0.copy()
 
+    Md = M if f.ndim == 1 else M[:, None]  # f may hold several right-hand sides as columns
+
     def smooth(x, sweeps):
         for _ in range(sweeps):
-            x = x + (f - A @ x) / M
+            x = x + (f - A @ x) / Md
         return x
 
     if l == nl - 1:

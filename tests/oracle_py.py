@@ -61,6 +61,18 @@ def lib():
         L.or_prec_afsai_G.restype = vp
         L.or_prec_afsai_G.argtypes = [vp]
         L.or_prec_apply.argtypes = [vp, F64P, F64P]
+        L.or_prec_num_hrc.restype = C.c_int
+        L.or_prec_num_hrc.argtypes = [vp, C.POINTER(C.c_double)]
+        L.or_prec_hrc_levels.restype = C.c_int
+        L.or_prec_hrc_levels.argtypes = [vp, C.c_int]
+        L.or_prec_hrc_level_A.restype = vp
+        L.or_prec_hrc_level_A.argtypes = [vp, C.c_int, C.c_int]
+        L.or_prec_hrc_level_P.restype = C.c_int
+        L.or_prec_hrc_level_P.argtypes = [vp, C.c_int, C.c_int, I32P, F64P]
+        L.or_prec_apply_hrc.argtypes = [vp, C.c_int, F64P, F64P]
+        L.or_prec_setup_w.restype = vp
+        L.or_prec_setup_w.argtypes = [vp, C.POINTER(abi.Params), F64P]
+        L.or_boot_random.argtypes = [C.c_int, C.c_ulonglong, F64P]
         L.or_fcg.restype = C.c_int
         L.or_fcg.argtypes = [vp, vp, C.POINTER(abi.Params), F64P, F64P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_cghs.restype = C.c_int
@@ -181,13 +193,50 @@ def make_p(M, w):
     return agg, pv, nc
 
 
+def boot_random(n, seed):
+    x = np.zeros(n)
+    lib().or_boot_random(n, seed, x)
+    return x
+
+
 class Prec:
-    def __init__(self, A, params):
+    def __init__(self, A, params, w=None):
+        """w: the smooth vector the matching starts from (one hierarchy; default 1)."""
         self.A = A
         self.params = params
-        self.h = lib().or_prec_setup(A.h, C.byref(params))
+        if w is None:
+            self.h = lib().or_prec_setup(A.h, C.byref(params))
+        else:
+            self.h = lib().or_prec_setup_w(A.h, C.byref(params), np.ascontiguousarray(w, np.float64))
         if not self.h:
             raise RuntimeError("oracle prec setup failed")
+
+    def num_hrc(self):
+        """(number of hierarchies, last estimated convergence ratio)"""
+        r = C.c_double()
+        n = lib().or_prec_num_hrc(self.h, C.byref(r))
+        return n, r.value
+
+    def hrc_levels(self, h):
+        return lib().or_prec_hrc_levels(self.h, h)
+
+    def hrc_level_A(self, h, l):
+        c = Csr(lib().or_prec_hrc_level_A(self.h, h, l), owned=False)
+        c.parent = self
+        return c
+
+    def hrc_level_P(self, h, l):
+        n, _, _ = self.hrc_level_A(h, l).dims()
+        agg = np.zeros(n, np.int32)
+        pv = np.zeros(n, np.float64)
+        lib().or_prec_hrc_level_P(self.h, h, l, agg, pv)
+        return agg, pv
+
+    def apply_hrc(self, h, r, u=None):
+        n, _, _ = self.A.dims()
+        u = np.zeros(n) if u is None else np.ascontiguousarray(u, np.float64).copy()
+        lib().or_prec_apply_hrc(self.h, h, np.ascontiguousarray(r, np.float64), u)
+        return u
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
