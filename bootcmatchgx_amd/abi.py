@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("autotune", C.c_int),
         ("check_every", C.c_int),
         ("fcg_z32", C.c_int),
+        ("post_galerkin", C.c_int),
     ]
 
 
@@ -128,4 +129,5 @@ def reference_defaults():
     p.autotune = 1
     p.check_every = 8
     p.fcg_z32 = 1
+    p.post_galerkin = 1
     return p

@@ -244,6 +244,23 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
     H.avg_cratio = nlv > 1 ? avc / (nlv - 1) : 1;
 }
 
+// A_l P_l of every level above the coarsest, values in FP32 only: with them the
+// first post-smoothing sweep forms f - A (x + P e) as r_pre - (A P) e
+static void make_level_product(Context* c, Hierarchy& H, int l) {
+    Level& lv = *H.L[l];
+    const DCsr& A = lv.mat();
+    DCsr P;
+    prolongator_csr(c, A.n, H.L[l + 1]->mat().n, lv.agg.get(), lv.pval.get(), P);
+    spgemm(c, A, P, lv.AP);
+    lv.AP.valf.reset(c, std::max(lv.AP.nnz, 1));
+    convert<double, float>(c, lv.AP.val.get(), lv.AP.valf.get(), lv.AP.nnz);
+    lv.AP.val.release();
+    lv.lanesAP = default_lanes(lv.AP.nnz, lv.AP.n);
+}
+static void make_level_products(Context* c, Hierarchy& H) {
+    for (int l = 0; l + 1 < H.levels(); l++) make_level_product(c, H, l);
+}
+
 static void make_fp32_copies(Context* c, Hierarchy& H) {
     for (auto& lp : H.L) {
         Level& l = *lp;
@@ -506,10 +523,31 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
             if (l + 1 == nl - 1) break;
         }
     }
-    if (xc) prolong_add<V>(c, A.n, k, lv.agg.get(), lvl_pval<V>(lv), xc, cur, R.stop);
     oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
-    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, p.postrelax_sweeps * coeff, false, k, R.stop,
-                           lv.chunkA, out64, outv);
+    int post = p.postrelax_sweeps * coeff;
+    if constexpr (std::is_same<V, float>::value) {
+        // the prolongation and the first post-smoothing sweep from A P in one pass:
+        // f - A (x + P e) = (f - A x) - (A P) e, the first term still in w.Res
+        if (xc && post >= 1 && lv.AP.valf) {
+            V* y = post == 1 && outv ? outv : oth;
+            SpmvArgs a;
+            a.n = lv.AP.n; a.row = lv.AP.row.get(); a.col = lv.AP.col.get(); a.val = lv.AP.valf.get();
+            a.nnz = lv.AP.nnz; a.x = xc; a.y = y; a.b = w.Res.get(); a.D = lvl_M<V>(lv); a.k = k;
+            a.lanes = lv.lanesAP; a.stop = R.stop;
+            galerkin_post_sweep<V>(c, a, lv.agg.get(), lvl_pval<V>(lv), cur);
+            if (post == 1) {
+                if (!out64) return y;
+                convert_stop<V, double>(c, y, out64, (long long)A.n * k, R.stop);  // the result widened
+                return nullptr;
+            }
+            cur = y;
+            oth = (cur == w.X.get()) ? w.X2.get() : w.X.get();
+            return jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, post - 1, false, k, R.stop, lv.chunkA,
+                                    out64, outv);
+        }
+    }
+    if (xc) prolong_add<V>(c, A.n, k, lv.agg.get(), lvl_pval<V>(lv), xc, cur, R.stop);
+    cur = jacobi_sweeps<V>(c, A, lv.lanesA, lvl_M<V>(lv), F, cur, oth, post, false, k, R.stop, lv.chunkA, out64, outv);
     return cur;
 }
 
@@ -521,6 +559,7 @@ void Prec::refresh_level0() {
             Level& l0 = *hrc(h).L[0];
             abs_row_sum(ctx, A->n, A->row.get(), A->val.get(), l0.M.get());
             if (l0.Mf) convert<double, float>(ctx, l0.M.get(), l0.Mf.get(), A->n);
+            if (l0.AP.valf) make_level_product(ctx, hrc(h), 0);  // A P with the new values of A
         }
     }
 }
@@ -719,6 +758,7 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
         bool f32 = p.precision == BCMGX_PRECISION_FP32;
         auto finish = [&](Hierarchy& H) {
             if (f32) make_fp32_copies(c, H);
+            if (f32 && p.post_galerkin) make_level_products(c, H);
             for (auto& lp : H.L) {
                 Level& l = *lp;
                 const DCsr& M = l.mat();

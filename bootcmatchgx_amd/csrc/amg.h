@@ -18,7 +18,8 @@ struct Level {
     DBuf<double> pval;
     DBuf<float> pvalf;
     DCsr R;                 // restrictor P^T (rows sorted by fine index)
-    int lanesA = 0, lanesR = 0;
+    DCsr AP;                // A P, FP32 values only (valf): the first post-smoothing sweep of an FP32 cycle
+    int lanesA = 0, lanesR = 0, lanesAP = 0;
     int chunkA = 0;         // columns per SpMM launch for the current batch width
     const DCsr& mat() const { return Aref ? *Aref : A; }
 };

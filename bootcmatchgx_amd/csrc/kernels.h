@@ -96,6 +96,13 @@ template <typename V>
 void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const V* xc, V* x, const int* stop);
 template <typename A, typename B>
 void convert_stop(Context* c, const A* s, B* d, long long n, const int* stop);
+// the first post-smoothing sweep of a level from its product A P: a describes
+// A P (n x nc) with a.x = the coarse correction e (nc x k), a.b = the
+// pre-smoothing residual r_pre = f - A xpre, a.D = the l1 row sums; writes
+// y = ((-(A P) e + r_pre) / D) + (p_i e[agg_i] + xpre), i.e. one l1-Jacobi sweep
+// on f from xpre + P e.  y must differ from xpre and b.
+template <typename V>
+void galerkin_post_sweep(Context* c, const SpmvArgs& a, const int* agg, const V* pval, const V* xpre);
 
 // the same with z (the SpMM's x) in FP32, widened exactly on load (four columns
 // per lane); 0 / false unless k is a multiple of 4 in [8, 64]

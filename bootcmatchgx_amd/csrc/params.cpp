@@ -104,6 +104,7 @@ int bcmgx_params_default(bcmgx_params* p) {
     p->autotune = 1;
     p->check_every = 8;
     p->fcg_z32 = 1;
+    p->post_galerkin = 1;
     return BCMGX_OK;
 }
 
@@ -169,6 +170,7 @@ int bcmgx_params_set(bcmgx_params* p, const char* key_c, const char* value_c) {
     if (key == "autotune") return seti(&p->autotune);
     if (key == "check_every") return seti(&p->check_every);
     if (key == "fcg_z32") return seti(&p->fcg_z32);
+    if (key == "post_galerkin") return seti(&p->post_galerkin);
     return BCMGX_OK;  // unknown keys are ignored (Params.cu:345-398)
 }
 

@@ -186,6 +186,9 @@ struct SpmmExtra {
     const double* r = nullptr;     // MODE 2: FCG residual
     const double* wold = nullptr;  // MODE 2: previous A-direction
     double* partials = nullptr;    // MODE 2: per-block dot partials
+    const int* agg = nullptr;      // MODE 4: each row's coarse index (x is the coarse correction e)
+    const void* pv = nullptr;      // MODE 4: the prolongator's value of each row (V*)
+    const void* xpre = nullptr;    // MODE 4: the pre-smoothed iterate (V*)
 };
 
 // Software-pipelined persistent k-vector SpMM, CPL consecutive columns per
@@ -205,6 +208,10 @@ struct SpmmExtra {
 //   MODE 2 (FP64, AX): w = A p and FCG's triple dot product (r.p, w.p, wold.p)
 //          per column (triple_innerproduct, src/op/triple_inner_product.cu:7-101),
 //          reduced per block in a fixed order into ex.partials[(blk*k + c)*3 + m].
+//   MODE 4: the first post-smoothing sweep from the product A P (the matrix
+//          here, n x nc) and the coarse correction e (x here): with
+//          z1 = p_i e[agg_i] + xpre_i (the prolongation), y = ((-t + b) / D) + z1,
+//          t = (A P) e, b the pre-smoothing residual.
 template <typename T, typename V, int KP, int CPL, int M, SpOp OP, int MODE, bool NT = false, typename XT = V>
 __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restrict__ row, const int* __restrict__ col,
                                               const T* __restrict__ val, const XT* __restrict__ x, V* __restrict__ y,
@@ -337,6 +344,10 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
             } else if constexpr (MODE == 3) {
                 *(VV*)(y + o) = t;
                 *(VV*)((V*)ex.x0 + o) = t / D[ri];
+            } else if constexpr (MODE == 4) {
+                VV e = __builtin_convertvector(*(const XV*)(x + (long long)ex.agg[ri] * k + cl * CPL), VV);
+                VV z1 = ((const V*)ex.pv)[ri] * e + *(const VV*)((const V*)ex.xpre + o);
+                *(VV*)(y + o) = ((-t + *(const VV*)(b + o)) / D[ri]) + z1;
             } else if constexpr (MODE == 2) {
                 *(VV*)(y + o) = t;
                 VD pw = __builtin_convertvector(*(const XV*)(x + o), VD);
@@ -1003,6 +1014,53 @@ void prolong_add(Context* c, int n, int k, const int* agg, const V* pval, const 
 }
 template void prolong_add<double>(Context*, int, int, const int*, const double*, const double*, double*, const int*);
 template void prolong_add<float>(Context*, int, int, const int*, const float*, const float*, float*, const int*);
+
+// the epilogue of the A P post-smoothing sweep on t = (A P) e already in y:
+// y = ((-t + b) / D) + (p_i e[agg_i] + xpre)
+template <typename V>
+__global__ void k_post_galerkin(long long nk, int k, const int* __restrict__ agg, const V* __restrict__ pval,
+                                const V* __restrict__ e, const V* __restrict__ xpre, const V* __restrict__ b,
+                                const V* __restrict__ D, V* __restrict__ y, const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nk) return;
+    long long i = o / k;
+    int c = (int)(o - i * k);
+    V z1 = pval[i] * e[(long long)agg[i] * k + c] + xpre[o];
+    y[o] = ((-y[o] + b[o]) / D[i]) + z1;
+}
+
+template <typename V>
+void galerkin_post_sweep(Context* c, const SpmvArgs& a, const int* agg, const V* pval, const V* xpre) {
+    if (a.n <= 0) return;
+    if (!a.row || (a.nnz > 0 && (!a.col || !a.val)) || !a.x || !a.y || !a.b || !a.D || !agg || !pval || !xpre ||
+        a.y == xpre || a.y == a.b)
+        throw Error(BCMGX_ERR_STATE, "galerkin_post_sweep: bad operand");
+    constexpr int CPL = 16 / (int)sizeof(V);
+    bool aligned = (((uintptr_t)a.x | (uintptr_t)a.y | (uintptr_t)a.b | (uintptr_t)xpre) & 15) == 0;
+    bool full = a.kchunk <= 0 || a.kchunk >= a.k;
+    if (a.k > 1 && full && aligned && spmmp_lanes(a.k, CPL)) {
+        SpmmExtra ex;
+        ex.agg = agg;
+        ex.pv = pval;
+        ex.xpre = xpre;
+        SpmvArgs b = a;
+        b.kchunk = a.kchunk == -2 || a.kchunk <= -16 ? a.kchunk : 0;
+        if (b.kchunk == -2) dispatch_spmmp<V, V, SpOp::AX, 4, true>(c, b, ex, 0);
+        else dispatch_spmmp<V, V, SpOp::AX, 4>(c, b, ex, 0);
+        return;
+    }
+    SpmvArgs b = a;
+    b.b = nullptr;
+    b.D = nullptr;
+    spmv<V, V>(c, SpOp::AX, b);  // t = (A P) e into y
+    long long nk = (long long)a.n * a.k;
+    k_post_galerkin<V><<<grid_for(nk, WG), WG, 0, c->stream>>>(nk, a.k, agg, pval, (const V*)a.x, xpre, (const V*)a.b,
+                                                              (const V*)a.D, (V*)a.y, a.stop);
+    BCMGX_LAUNCHED();
+}
+template void galerkin_post_sweep<double>(Context*, const SpmvArgs&, const int*, const double*, const double*);
+template void galerkin_post_sweep<float>(Context*, const SpmvArgs&, const int*, const float*, const float*);
 
 template <typename A, typename B>
 __global__ void k_convert_stop(const A* __restrict__ s, B* __restrict__ d, long long n, const int* __restrict__ stop) {

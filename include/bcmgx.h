@@ -91,6 +91,10 @@ typedef struct bcmgx_params {
     int fcg_z32;            /* "fcg_z32": with an FP32 hierarchy, FCG keeps the cycle's result z in FP32 and
                                widens it exactly inside its kernels (fewer bytes; the dot products are blocked
                                differently, so iterates may differ in the last bits; default 1) */
+    int post_galerkin;      /* "post_galerkin": with an FP32 hierarchy, the first post-smoothing sweep of each
+                               level takes its residual as r_pre - (A P) e, from the pre-smoothing residual and
+                               the product A P kept from setup, in place of f - A (x + P e) (the same value in
+                               exact arithmetic, from the sparser A P; default 1) */
 } bcmgx_params;
 
 /* Defaults of src/config/Params.h:238-291. */
@@ -138,7 +142,8 @@ int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val);
  * used by the autograd shim when the values it differentiates change.  A
  * preconditioner set up on A keeps its hierarchy: its level 0 is A itself and
  * sees the new values, and the level-0 divisors are recomputed from them (the
- * l1 row sums of BCMG, the l1 diagonal of one-level l1-Jacobi); the coarse
+ * l1 row sums of BCMG, the l1 diagonal of one-level l1-Jacobi), as is the
+ * level-0 product A P of an FP32 hierarchy with post_galerkin; the coarse
  * levels, the prolongators and an AFSAI factor are not rebuilt. */
 int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem);
 

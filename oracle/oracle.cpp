@@ -505,6 +505,8 @@ struct Level {
     std::vector<double> pval;
     Csr R;                     // restrictor to the next level
     std::vector<float> Rf, Pf;
+    Csr AP;                    // A P (FP32 cycle with post_galerkin): structure, FP64 values
+    std::vector<float> APf;    // its FP32 values
 };
 
 struct Hierarchy {
@@ -602,6 +604,13 @@ void build_hierarchy(const Csr& A0, const bcmgx_params& p, Hierarchy& H, const s
             l.Rf.assign(l.R.val.begin(), l.R.val.end());
             l.Pf.assign(l.pval.begin(), l.pval.end());
         }
+        // the first post-smoothing sweep's operator A P of every level above the coarsest
+        if (p.post_galerkin)
+            for (int i = 0; i + 1 < nl; i++) {
+                Level& l = H.L[i];
+                l.AP = spgemm(l.A, p_from_map(l.A.n, H.L[i + 1].A.n, l.agg, l.pval));
+                l.APf.assign(l.AP.val.begin(), l.AP.val.end());
+            }
     }
 }
 
@@ -683,7 +692,22 @@ void gamg_cycle(const Hierarchy& H, const bcmgx_params& p, CycleWork<V>& W, int 
         t += ((V)1.0 * pv[i]) * xc[lv.agg[i]];
         x[i] = t + ((V)1.0 * x[i]);
     }
-    jacobi<V>(p.postrelax_sweeps * coeff, lv.A, *Aval[l], Mv[l]->data(), W.F[l].data(), W.X[l], W.X2[l]);
+    int post = p.postrelax_sweeps * coeff;
+    if (sizeof(V) == sizeof(float) && post >= 1 && !lv.APf.empty() && (nl > 2 || p.relaxnumber_coarse > 0)) {
+        // FP32 cycle, post_galerkin: the first post sweep takes f - A x = res - (A P) x_{l+1},
+        // res = f - A x_pre (the pre-smoothing residual), x = x_pre + P x_{l+1}
+        const V* D = Mv[l]->data();
+        std::vector<V>& x2 = W.X2[l];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < lv.A.n; i++) {
+            V t = 0;
+            for (int j = lv.AP.row[i]; j < lv.AP.row[i + 1]; j++) t += (V)lv.APf[j] * xc[lv.AP.col[j]];
+            x2[i] = ((-t + res[i]) / D[i]) + x[i];
+        }
+        std::swap(W.X[l], W.X2[l]);
+        post -= 1;
+    }
+    jacobi<V>(post, lv.A, *Aval[l], Mv[l]->data(), W.F[l].data(), W.X[l], W.X2[l]);
 }
 
 }  // namespace

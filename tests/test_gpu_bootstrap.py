@@ -42,11 +42,13 @@ def oparams(p):
     return q
 
 
-def check_hierarchies(P, op, exact_first=True):
+def check_hierarchies(P, op, exact_first=True, tol=1e-10, ratio_tol=1e-8):
+    """tol: later hierarchies' values and prolongators (relative to the largest);
+    ratio_tol: the estimated convergence ratio (relative)."""
     nh, ratio = P.num_hrc()
     onh, oratio = op.num_hrc()
     assert nh == onh
-    assert abs(ratio - oratio) <= 1e-8 * max(oratio, 1e-300)
+    assert abs(ratio - oratio) <= ratio_tol * max(oratio, 1e-300)
     for h in range(nh):
         assert P.levels(h) == op.hrc_levels(h), h
         for l in range(P.levels(h)):
@@ -56,12 +58,12 @@ def check_hierarchies(P, op, exact_first=True):
             if h == 0 and exact_first:
                 assert np.array_equal(g.data, o.data), (h, l)
             else:
-                assert np.all(np.abs(g.data - o.data) <= 1e-10 * np.abs(o.data).max()), (h, l)
+                assert np.all(np.abs(g.data - o.data) <= tol * np.abs(o.data).max()), (h, l)
             if l < P.levels(h) - 1:
                 ga, gv = P.level_prolongator(l, h)
                 oa_, ov = op.hrc_level_P(h, l)
                 assert np.array_equal(ga, oa_), (h, l)
-                assert np.all(np.abs(gv - ov) <= 1e-10 * np.abs(ov).max()), (h, l)
+                assert np.all(np.abs(gv - ov) <= tol * np.abs(ov).max()), (h, l)
 
 
 @pytest.mark.parametrize("comp", COMPS)
@@ -113,13 +115,19 @@ def test_bootstrap_batched_columns_equal_single_solves(ctx, comp):
 @pytest.mark.parametrize("comp", COMPS)
 def test_bootstrap_fp32_hierarchies(ctx, comp):
     """FP32 hierarchy values and cycles under the FP64 FCG: the oracle's FP32 cycle
-    makes the same hierarchies; solves agree in iterations (+-1) and to 1e-7."""
+    makes the same hierarchies; solves agree in iterations (+-1) and to 1e-7.  The
+    smooth vectors of the later hierarchies come from FP32 cycles, whose sums the GPU
+    groups differently (FP32 rounding 6e-8 per operation): the estimated ratio is
+    checked to 1e-6 and the later hierarchies' values to 1e-5, their aggregates
+    exactly; the first hierarchy is bit-exact."""
     oa = O.diffusion7p(16, 15, 14, 1, 0.5, 2.0, 4)
     A = dev(ctx, oa)
     p = params(comp, max_hrc=2, precision="FP32")
     P = bx.prec_setup(ctx, A, p)
     op = O.Prec(oa, oparams(p))
-    check_hierarchies(P, op)
+    nh, ratio = P.num_hrc()
+    print(f"{comp}: estimated ratio {ratio:.10f} (oracle {op.num_hrc()[1]:.10f})")
+    check_hierarchies(P, op, tol=1e-5, ratio_tol=1e-6)
     b = np.random.default_rng(11).normal(size=A.n)
     x, out = bx.solve(ctx, A, b, None, p, P)
     xo, oi = O.fcg(oa, op, oparams(p), b)

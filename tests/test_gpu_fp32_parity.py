@@ -38,17 +38,31 @@ def oparams(p):
     return q
 
 
-@pytest.mark.parametrize("k", [1, 8, 32])
-def test_fp32_cycle_matches_oracle_fp32_cycle(ctx, k):
-    """27-point 42^3 (74,088 rows: level 0 takes the autotuned kernels)."""
+CYCLES = [
+    (1, dict(post_galerkin=1)),
+    (8, dict(post_galerkin=1)),
+    (32, dict(post_galerkin=1)),
+    (32, dict(post_galerkin=0)),
+    (1, dict(post_galerkin=0)),
+    (8, dict(post_galerkin=1, cycle_type="W_CYCLE", postrelax_sweeps=2)),
+    (1, dict(post_galerkin=1, cycle_type="VARIABLE_V_CYCLE", postrelax_sweeps=2)),
+]
+
+
+@pytest.mark.parametrize("k,kw", CYCLES)
+def test_fp32_cycle_matches_oracle_fp32_cycle(ctx, k, kw):
+    """27-point 42^3 (74,088 rows: level 0 takes the autotuned kernels).  With
+    post_galerkin the first post-smoothing sweep of each level runs on A P (k >= 8:
+    the pipelined SpMM's fused epilogue; k = 1: the product, then the epilogue)."""
     oa = O.laplacian3d(27, 42, 42, 42)
     A = dev(ctx, oa)
-    p = params("BCMG", prerelax_sweeps=1, postrelax_sweeps=1, precision="FP32")
+    kw = dict(dict(prerelax_sweeps=1, postrelax_sweeps=1), **kw)
+    p = params("BCMG", precision="FP32", **kw)
     P = bx.prec_setup(ctx, A, p)
     R = np.random.default_rng(31).normal(size=(A.n, k))
     got = P.apply(R).reshape(A.n, k)
     op = O.Prec(oa, oparams(p))
-    p64 = params("BCMG", prerelax_sweeps=1, postrelax_sweeps=1)
+    p64 = params("BCMG", **kw)
     op64 = O.Prec(oa, oparams(p64))
     worst, vs64 = 0.0, 0.0
     for j in range(k):
@@ -56,7 +70,7 @@ def test_fp32_cycle_matches_oracle_fp32_cycle(ctx, k):
         worst = max(worst, np.linalg.norm(got[:, j] - ref) / np.linalg.norm(ref))
         r64 = op64.apply(R[:, j].copy())
         vs64 = max(vs64, np.linalg.norm(got[:, j] - r64) / np.linalg.norm(r64))
-    print(f"k={k}: FP32 cycle vs oracle FP32 cycle {worst:.2e}, vs oracle FP64 cycle {vs64:.2e}")
+    print(f"k={k} {kw}: FP32 cycle vs oracle FP32 cycle {worst:.2e}, vs oracle FP64 cycle {vs64:.2e}")
     assert worst <= 1e-6
     assert 1e-9 < vs64 <= 1e-4  # it ran in FP32, and stays the FP64 operator to FP32 accuracy
     P.finalize()

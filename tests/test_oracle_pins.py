@@ -92,6 +92,27 @@ def test_fp32_cycle_equals_dense_restatement_to_fp32_rounding():
     assert err > 0  # it really ran in FP32
 
 
+@pytest.mark.parametrize("cycle,kw", CYCLES)
+def test_fp32_post_galerkin_cycle_equals_dense_restatement(cycle, kw):
+    """The FP32 cycle whose first post-smoothing sweep takes its residual from the
+    pre-smoothing residual and the product A P (f - A (x + P e) = (f - A x) - (A P) e)
+    is the same operator as the plain cycle: both agree with the dense restatement to
+    FP32 rounding, and the two FP32 paths really differ (they round differently)."""
+    A = diffusion(10, 9, 8, 14)
+    base = dict(cycle_type=abi.CYCLE[cycle + "_CYCLE"], aggrsweeps=0, max_coarse_size=12, precision=1, **kw)
+    pg = O.Prec(A, params(post_galerkin=1, **base))
+    plain = O.Prec(A, params(post_galerkin=0, **base))
+    H = DenseHierarchy.from_oracle(A, pg)
+    assert H.levels() >= 3
+    r = np.random.default_rng(7).normal(size=H.A[0].shape[0])
+    ref = np_cycle(H, params(**base), 0, r)
+    u_pg, u_plain = pg.apply(r), plain.apply(r)
+    nr = np.linalg.norm(ref)
+    assert np.linalg.norm(u_pg - ref) <= 2e-5 * nr
+    assert np.linalg.norm(u_plain - ref) <= 2e-5 * nr
+    assert np.linalg.norm(u_pg - u_plain) > 0
+
+
 def test_two_grid_with_exact_coarse_solve_is_closed_form():
     """Two levels and enough coarsest sweeps to converge: the cycle is the two-grid
     method whose error propagation is E = S^nu2 (I - P A_c^{-1} P^T A) S^nu1, with
