@@ -235,11 +235,17 @@ def test_solve_matches_reference_fixture(golden, pts, prec):
 
 
 @pytest.mark.parametrize("pts", [7, 27])
-def test_solve_matches_block_fixture(golden, pts):
-    A = O.laplacian3d(pts, 10, 10, 10, 2, 1, 1)
+@pytest.mark.parametrize("grid", [(2, 1, 1), (2, 2, 1), (2, 2, 2)])
+def test_solve_matches_block_fixture(golden, pts, grid):
+    """The P x Q x R block-ordered problems (each 10^3 subdomain one block,
+    laplacian.cu:14-25) against the reference's solution files."""
+    P, Q, R = grid
+    A = O.laplacian3d(pts, 10, 10, 10, P, Q, R)
+    n = 1000 * P * Q * R
     p = fcg_params("BCMG")
-    x, info = O.fcg(A, O.Prec(A, p), p, np.ones(2000))
-    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_2x1x1_sol.txt"))
+    x, info = O.fcg(A, O.Prec(A, p), p, np.ones(n))
+    ref = np.loadtxt(os.path.join(golden, f"lap3d_{pts}p_10x10x10_{P}x{Q}x{R}_sol.txt"))
+    assert info["retv"] == 0
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-5
 
 
