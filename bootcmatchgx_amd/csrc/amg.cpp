@@ -25,14 +25,16 @@ namespace bcmgx {
 int tune_lanes(Context* c, const DCsr& A, bool fp32) {
     if (A.n < 65536) return default_lanes(A.nnz, A.n);
     if (fp32 && !A.valf) throw Error(BCMGX_ERR_STATE, "tune_lanes: FP32 values not built");
-    std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d");
+    std::string sig = std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d") + (A.tiled() ? "t" : "");
     auto it = c->tuned_lanes.find(sig);
     if (it != c->tuned_lanes.end()) return it->second;
     int base = default_lanes(A.nnz, A.n);
     // vector kernels (L lanes per row), CSR-stream kernels (-TPR lanes per row
     // reduction), their paired-load form (-200 - TPR), and both with rows dealt
-    // to the XCDs in super-blocks of 32 blocks (-6000 - ...)
-    int cands[10] = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8, -204, -208, -6004, -6204};
+    // to the XCDs in super-blocks of 32 blocks (-6000 - ...); with a row-tile
+    // index the tiled kernel (-404), also in super-blocks of 16 and 64 (-5404, -7404)
+    std::vector<int> cands = {std::max(1, base / 2), base, std::min(64, base * 2), -2, -4, -8, -204, -208, -6004, -6204};
+    if (A.tiled()) cands.insert(cands.end(), {-404, -5404, -7404});
     hipEvent_t e0, e1;
     BCMGX_HIP(hipEventCreate(&e0));
     BCMGX_HIP(hipEventCreate(&e1));
@@ -44,6 +46,7 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
         for (int L : cands) {
             SpmvArgs a;
             a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.valf.get(); a.nnz = A.nnz;
+            set_tiles(a, A);
             a.x = x.get(); a.y = y.get(); a.lanes = L;
             spmv<float, float>(c, SpOp::AX, a);
             BCMGX_HIP(hipEventRecord(e0, c->stream));
@@ -60,6 +63,7 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
         for (int L : cands) {
             SpmvArgs a;
             a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+            set_tiles(a, A);
             a.x = x.get(); a.y = y.get(); a.lanes = L;
             spmv<double, double>(c, SpOp::AX, a);
             BCMGX_HIP(hipEventRecord(e0, c->stream));
@@ -79,7 +83,8 @@ int tune_lanes(Context* c, const DCsr& A, bool fp32) {
 
 int cached_lanes(Context* c, const DCsr& A, bool fp32) {
     if (A.n >= 65536) {
-        auto it = c->tuned_lanes.find(std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d"));
+        auto it = c->tuned_lanes.find(std::to_string(A.n) + ":" + std::to_string(A.nnz) + (fp32 ? "f" : "d") +
+                                      (A.tiled() ? "t" : ""));
         if (it != c->tuned_lanes.end()) return it->second;
     }
     return default_lanes(A.nnz, A.n);
@@ -401,6 +406,7 @@ static V* jacobi_sweeps(Context* c, const DCsr& A, int lanes, const V* M, const 
     for (; done < s; done++) {
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = cur; a.y = oth; a.b = F; a.D = M; a.k = k; a.lanes = lanes; a.stop = stop; a.kchunk = chunk;
         bool last = done + 1 == s;
         if (last && out64) {
@@ -496,6 +502,7 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
     {
         SpmvArgs a;
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = cur; a.y = w.Res.get(); a.b = F; a.k = k; a.lanes = lv.lanesA; a.stop = R.stop; a.kchunk = lv.chunkA;
         spmv<V, V>(c, SpOp::RES, a);
     }
@@ -659,6 +666,7 @@ void Prec::apply(const double* r, double* u, int k, bool x_zero, const int* stop
             if (!x_zero) {
                 SpmvArgs a;
                 a.n = A->n; a.row = A->row.get(); a.col = A->col.get(); a.val = A->val.get(); a.nnz = A->nnz;
+                set_tiles(a, *A);
                 a.x = u; a.y = tmp.get(); a.b = r; a.k = k; a.lanes = cached_lanes(c, *A, false); a.stop = stop;
                 spmv<double, double>(c, SpOp::RES, a);
                 res = tmp.get();
@@ -710,6 +718,7 @@ static void bootstrap(Context* c, const DCsr& A, Prec* pr, Build build_next) {
         auto anorm2 = [&](int slot) {
             SpmvArgs a;
             a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+            set_tiles(a, A);
             a.x = x.get(); a.y = ax.get(); a.lanes = cached_lanes(c, A, false);
             spmv<double, double>(c, SpOp::AX, a);
             block_dots(c, n, 1, x.get(), ax.get(), 0, part.get(), nblk, dots.get() + slot);
@@ -737,6 +746,9 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
     pr->A = &A;
     pr->p = p;
     pr->ptype = p.preconditioner;
+    // the row-tile index of the caller's matrix (its pattern only: built once, kept
+    // with the matrix) serves the one-column products with it
+    if (A.n >= 65536 && p.autotune) build_tile_index(c, const_cast<DCsr&>(A));
     switch (p.preconditioner) {
     case BCMGX_PREC_NONE:
         break;
@@ -794,6 +806,7 @@ static void fcg_iteration_z32(Context* c, const DCsr& A, Prec* prec, SolveWork& 
     prec->apply_entry_f32(s.zf.get(), s.k, stop);
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+    set_tiles(a, A);
     a.x = s.zf.get(); a.y = s.w.get(); a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
     BCMGX_CHECK(spmm_dots_z32(c, a, s.r.get(), s.q.get(), s.fb.partials, s.z32_blocks, s.fb.all_done), BCMGX_ERR_STATE,
                 "fused FP32-z product not applicable");
@@ -815,6 +828,7 @@ static void fcg_iteration(Context* c, const DCsr& A, Prec* prec, SolveWork& s, i
     prec->apply(s.r.get(), P0, s.k, true, stop, s.fused);  // P0 zero-filled in the reference (FCG.cu:106,119)
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+    set_tiles(a, A);
     a.x = P0; a.y = W0; a.k = s.k; a.lanes = s.lanesA; a.stop = stop; a.kchunk = s.chunkA;
     if (s.mv_blocks > 0 && spmm_dots(c, a, s.r.get(), W1, s.fb.partials, s.mv_blocks, s.fb.all_done)) {
         fcg_scalars3(c, s.fb, s.mv_blocks, idx);
@@ -893,6 +907,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = s.X.get(); a.y = s.r.get(); a.b = s.B.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         a.stop = io ? xzero : nullptr;
         spmv<double, double>(c, SpOp::RES, a);  // r = b - A x
@@ -905,6 +920,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         prec->apply_entry_f32(s.zf.get(), k, nullptr);
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = s.zf.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         BCMGX_CHECK(spmm_dots_z32(c, a, s.r.get(), s.r.get(), s.fb.partials, s.z32_blocks, nullptr), BCMGX_ERR_STATE,
                     "fused FP32-z product not applicable");
@@ -914,6 +930,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
         prec->apply(s.r.get(), s.v.get(), k, true, nullptr, s.fused);
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = s.v.get(); a.y = s.w.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA;
         spmv<double, double>(c, SpOp::AX, a);  // w = A v
         fcg_init_dots(c, s.fb, s.r.get(), s.v.get(), s.w.get());
@@ -1010,6 +1027,7 @@ SolveResult cghs_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params
     auto spmm = [&](const double* xin, double* yout, SpOp op, const int* st) {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = xin; a.y = yout; a.b = s.B.get(); a.k = k; a.lanes = s.lanesA; a.kchunk = s.chunkA; a.stop = st;
         spmv<double, double>(c, op, a);
     };

@@ -562,13 +562,15 @@ int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, do
     API_BEGIN
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
-    const DCsr& M = A->A;
+    DCsr& M = A->A;
+    if (lanes < 0 && (-lanes) % 1000 >= 400) build_tile_index(c, M);  // the row-tile kernel asked for
     DBuf<double> X(c, std::max(M.m, 1)), Y(c, std::max(M.n, 1));
     hipMemcpyKind kin = mem == BCMGX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     BCMGX_HIP(hipMemcpyAsync(X.get(), x, sizeof(double) * M.m, kin, c->stream));
     if (op != 1) BCMGX_HIP(hipMemcpyAsync(Y.get(), y, sizeof(double) * M.n, kin, c->stream));
     SpmvArgs a;
     a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.val = M.val.get(); a.nnz = M.nnz;
+    set_tiles(a, M);
     a.x = X.get(); a.y = Y.get(); a.b = Y.get(); a.alpha = alpha; a.beta = beta; a.lanes = lanes;
     SpOp o = op == 1 ? SpOp::AX : (op == 2 ? SpOp::RES : SpOp::AXPBY);
     if (lanes == 0) a.lanes = tune_lanes(c, M, false);
@@ -663,6 +665,7 @@ int bcmgx_spmm(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, in
     if (op != 1) stage_in(c, y, ld, mem, M.n, k, Y.get(), tmp);
     SpmvArgs a;
     a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.val = M.val.get(); a.nnz = M.nnz;
+    set_tiles(a, M);
     a.x = X.get(); a.y = Y.get(); a.b = Y.get(); a.alpha = alpha; a.beta = beta; a.k = k; a.kchunk = kchunk;
     a.lanes = k == 1 ? tune_lanes(c, M, false) : 0;
     SpOp o = op == 1 ? SpOp::AX : (op == 2 ? SpOp::RES : SpOp::AXPBY);
@@ -695,9 +698,14 @@ int bcmgx_time_spmv(bcmgx_context ctx, bcmgx_matrix A, int precision, int k, int
     size_t nx = (size_t)std::max(M.m, 1) * k, ny = (size_t)std::max(M.n, 1) * k;
     SpmvArgs a;
     a.n = M.n; a.row = M.row.get(); a.col = M.col.get(); a.nnz = M.nnz; a.k = k;
+    set_tiles(a, M);
     if (precision == 1 && !M.valf) {  // the FP32 copy must exist before any FP32 launch (tuning included)
         M.valf.reset(c, std::max(M.nnz, 1));
         convert<double, float>(c, M.val.get(), M.valf.get(), M.nnz);
+    }
+    if (k == 1 && M.n >= 65536) {
+        build_tile_index(c, M);
+        set_tiles(a, M);
     }
     if (k == 1) a.lanes = lanes != 0 ? lanes : tune_lanes(c, M, precision == 1);
     else a.kchunk = (lanes >= -2 || (lanes <= -16 && lanes >= -28)) ? lanes : 0;
@@ -749,6 +757,7 @@ int bcmgx_time_smoother(bcmgx_context ctx, bcmgx_matrix A, int precision, int k,
     size_t nk = (size_t)n * k;
     SpmvArgs a;
     a.n = n; a.row = M.row.get(); a.col = M.col.get(); a.nnz = M.nnz; a.k = k;
+    set_tiles(a, M);
     // the launch shape the V-cycle picks for this operator and batch width
     if (k == 1) a.lanes = tune_lanes(c, M, f32);
     else a.kchunk = tune_chunk(c, M, f32, k);

@@ -175,11 +175,24 @@ void Context::d2h(T* dst, const T* src, size_t n) {
 // ---------------------------------------------------------------------------
 // Device CSR: int32 row pointers/columns, FP64 values, optional FP32 copy.
 // ---------------------------------------------------------------------------
+// Row-tile index of the pattern (built on demand, kept with the matrix): for
+// every tile of TILE_ROWS consecutive rows the sorted distinct columns its
+// entries touch (tucol[tu[t] .. tu[t+1])) and, per entry, the 16-bit position
+// of its column in that list (tlidx).  tu[t+1] == tu[t] for a tile with
+// entries marks it unindexed (too many entries or columns): it is read by
+// column index.
+constexpr int TILE_ROWS = 64;
+constexpr int TILE_MAX_ENTRIES = 2048;
+constexpr int TILE_MAX_COLS = 1024;
+
 struct DCsr {
     int n = 0, m = 0, nnz = 0;
     DBuf<int> row, col;
     DBuf<double> val;
     DBuf<float> valf;  // FP32 copy for the mixed-precision cycle (may be empty)
+    DBuf<int> tu, tucol;            // row-tile index (empty until built)
+    DBuf<unsigned short> tlidx;
+    bool tiled() const { return (bool)tu; }
     void alloc(Context* c, int n_, int m_, int nnz_) {
         n = n_; m = m_; nnz = nnz_;
         row.reset(c, (size_t)n + 1);

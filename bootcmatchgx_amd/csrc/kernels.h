@@ -67,7 +67,18 @@ struct SpmvArgs {
     const int* stop = nullptr;  // device flag: skip the launch's work when *stop != 0
     double* y64 = nullptr;      // JACOBI with V = float: write the result widened to FP64 here (y is scratch)
     void* x0 = nullptr;         // AX: also write y / D here (V*; a first l1-Jacobi sweep from zero on y)
+    const int* tu = nullptr;    // the matrix's row-tile index (DCsr::tu, tucol, tlidx), or null
+    const int* tucol = nullptr;
+    const unsigned short* tlidx = nullptr;
 };
+// the row-tile index of A into a (k = 1 launches whose lanes code asks for the tiled kernel use it)
+inline void set_tiles(SpmvArgs& a, const DCsr& A) {
+    a.tu = A.tu.get();
+    a.tucol = A.tucol.get();
+    a.tlidx = A.tlidx.get();
+}
+// build A's row-tile index (once: a built index is kept)
+void build_tile_index(Context* c, DCsr& A);
 // T = matrix value type, V = vector type: (double,double), (float,float), (float,double)
 template <typename T, typename V>
 void spmv(Context* c, SpOp op, const SpmvArgs& a);

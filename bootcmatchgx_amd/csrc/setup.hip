@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 
 namespace bcmgx {
 
@@ -1227,6 +1228,102 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     k_spgemm_compact<<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff.get(), C.row.get(), tcol.get(), tval.get(), C.col.get(),
                                                                 C.val.get());
     BCMGX_LAUNCHED();
+}
+
+// ------------------------------------------------------- row-tile index ---
+// One workgroup per tile of TILE_ROWS rows: its columns sorted in LDS (bitonic,
+// padded to a power of two), the distinct ones counted (phase 0) or listed and
+// every entry given its position in the list (phase 1).  A tile is indexed when
+// it has entries, its paired loads from an even start fit one pass
+// (cnt + (base & 1) <= TILE_MAX_ENTRIES) and it has at most TILE_MAX_COLS
+// distinct columns; others count 0.
+__global__ __launch_bounds__(WG) void k_tile_cols(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                  int* __restrict__ ucnt, const int* __restrict__ tu,
+                                                  int* __restrict__ tucol, unsigned short* __restrict__ tlidx, int phase) {
+    __shared__ int keys[TILE_MAX_ENTRIES];
+    __shared__ int uniq[TILE_MAX_COLS];
+    __shared__ int wsum[WG / 64];
+    int t = blockIdx.x, tid = threadIdx.x;
+    int r0 = t * TILE_ROWS;
+    int rend = r0 + TILE_ROWS < n ? r0 + TILE_ROWS : n;
+    int base = row[r0], cnt = row[rend] - base;
+    bool fits = cnt > 0 && cnt + (base & 1) <= TILE_MAX_ENTRIES;
+    if (phase == 1 && tu[t + 1] == tu[t]) return;  // not indexed
+    if (!fits) {
+        if (phase == 0 && tid == 0) ucnt[t] = 0;
+        return;
+    }
+    int P = 1;
+    while (P < cnt) P <<= 1;
+    for (int p = tid; p < P; p += WG) keys[p] = p < cnt ? col[base + p] : INT_MAX;
+    __syncthreads();
+    for (int kk = 2; kk <= P; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int p = tid; p < P; p += WG) {
+                int q = p ^ j;
+                if (q > p) {
+                    int a = keys[p], b = keys[q];
+                    bool up = (p & kk) == 0;
+                    if ((a > b) == up) {
+                        keys[p] = b;
+                        keys[q] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // distinct columns in order: heads of runs, ranked by a block scan per chunk of WG
+    int lane = tid & 63, w = tid >> 6;
+    int carry = 0;
+    for (int c0 = 0; c0 < cnt; c0 += WG) {
+        int p = c0 + tid;
+        bool head = p < cnt && (p == 0 || keys[p] != keys[p - 1]);
+        unsigned long long m = __ballot(head);
+        int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int off = carry;
+        for (int q = 0; q < w; q++) off += wsum[q];
+        if (phase == 1 && head && off + below < TILE_MAX_COLS) uniq[off + below] = keys[p];
+        int tot = 0;
+        for (int q = 0; q < WG / 64; q++) tot += wsum[q];
+        carry += tot;
+        __syncthreads();
+    }
+    int U = carry <= TILE_MAX_COLS ? carry : 0;
+    if (phase == 0) {
+        if (tid == 0) ucnt[t] = U;
+        return;
+    }
+    int u0 = tu[t];
+    for (int u = tid; u < U; u += WG) tucol[u0 + u] = uniq[u];
+    for (int j = tid; j < cnt; j += WG) {
+        int c = col[base + j];
+        int lo = 0, hi = U - 1;  // uniq[lo] <= c <= uniq[hi], c is in the list
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (uniq[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+        tlidx[base + j] = (unsigned short)lo;
+    }
+}
+
+void build_tile_index(Context* c, DCsr& A) {
+    if (A.tiled() || A.n <= 0) return;
+    int nt = (A.n + TILE_ROWS - 1) / TILE_ROWS;
+    DBuf<int> cnt(c, nt), tu(c, (size_t)nt + 1);
+    k_tile_cols<<<nt, WG, 0, c->stream>>>(A.n, A.row.get(), A.col.get(), cnt.get(), nullptr, nullptr, nullptr, 0);
+    BCMGX_LAUNCHED();
+    long long tot = scan_exclusive(c, cnt.get(), tu.get(), nt);
+    DBuf<int> tucol(c, (size_t)std::max<long long>(tot, 1));
+    DBuf<unsigned short> tlidx(c, (size_t)std::max(A.nnz, 2));
+    k_tile_cols<<<nt, WG, 0, c->stream>>>(A.n, A.row.get(), A.col.get(), cnt.get(), tu.get(), tucol.get(), tlidx.get(), 1);
+    BCMGX_LAUNCHED();
+    A.tu = std::move(tu);
+    A.tucol = std::move(tucol);
+    A.tlidx = std::move(tlidx);
 }
 
 // ------------------------------------------------------ adjoint gradient ---

@@ -700,6 +700,115 @@ __global__ __launch_bounds__(WG) void k_spmv_stream2(int n, long long nnz, const
     }
 }
 
+// CSR-stream SpMV over the matrix's row-tile index: a workgroup owns one tile of
+// TILE_ROWS = 64 rows (4 lanes per row in the reduction).  It loads the x
+// entries of the tile's distinct columns once into LDS (the columns sorted, so
+// neighbouring lanes read neighbouring x), streams the (value, 16-bit local
+// column) pairs, forms each product from LDS and reduces the rows as
+// k_spmv_stream2 does (same products, same order: the same bits).  Fewer bytes
+// than CSR (2-byte column positions, plus the distinct-column list: about 9
+// per row on the 27-point grid) and a handful of coalesced x loads per tile
+// instead of one gather per entry.  A tile the index leaves out is read by
+// column index from global memory.
+template <typename T, typename V, SpOp OP>
+__global__ __launch_bounds__(WG) void k_spmv_tiled(int n, long long nnz, const int* __restrict__ row,
+                                                   const int* __restrict__ col, const T* __restrict__ val,
+                                                   const int* __restrict__ tu, const int* __restrict__ tucol,
+                                                   const unsigned short* __restrict__ tlidx, const V* __restrict__ x,
+                                                   V* __restrict__ y, const V* __restrict__ b, const V* __restrict__ D,
+                                                   V alpha, V beta, const int* __restrict__ stop, int sb) {
+    constexpr int TPR = WG / TILE_ROWS, RB = TILE_ROWS, UP = 4, XP = TILE_MAX_COLS / WG;
+    static_assert(2 * UP * WG == TILE_MAX_ENTRIES, "one pass of paired loads covers an indexed tile");
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    typedef unsigned short U2 __attribute__((ext_vector_type(2)));
+    if (stop && *stop) return;
+    __shared__ V prod[TILE_MAX_ENTRIES];
+    __shared__ V xs[TILE_MAX_COLS];
+    __shared__ int rp[RB + 1];
+    int blk = xcd_order(blockIdx.x, gridDim.x, sb);
+    int r0 = blk * RB;
+    int tid = threadIdx.x;
+    int rend = r0 + RB < n ? r0 + RB : n;
+    int base = row[r0], cnt = row[rend] - base;
+    int u0 = tu[blk], U = tu[blk + 1] - u0;
+    int myrp = 0;
+    if (tid < RB) myrp = row[r0 + tid < n ? r0 + tid : n];
+    bool indexed = U > 0;  // an indexed tile has entries and at most TILE_MAX_* of both
+    if (indexed) {
+        // x of the distinct columns (index clamped into the list: every load unconditional and in flight)
+        int uc[XP];
+#pragma unroll
+        for (int p = 0; p < XP; p++) {
+            int u = tid + p * WG;
+            uc[p] = tucol[u0 + (u < U ? u : U - 1)];
+        }
+        V xv[XP];
+#pragma unroll
+        for (int p = 0; p < XP; p++) xv[p] = x[uc[p]];
+        // the tile's (value, local column) pairs from an even-aligned start
+        int abase = base & ~1, acnt = base + cnt - abase;
+        const T2* v2 = (const T2*)(val + abase);
+        const U2* l2 = (const U2*)(tlidx + abase);
+        T2 a[UP];
+        U2 ll[UP];
+#pragma unroll
+        for (int q = 0; q < UP; q++) {
+            int pp = tid + q * WG;
+            a[q] = (T2)(T)0;
+            ll[q] = (U2)0;
+            if (2 * pp < acnt) {
+                long long g = (long long)abase + 2 * pp;
+                if (g + 1 < nnz) {
+                    a[q] = __builtin_nontemporal_load(v2 + pp);
+                    ll[q] = __builtin_nontemporal_load(l2 + pp);
+                } else {  // the matrix's last entry, unpaired
+                    a[q].x = val[g];
+                    ll[q].x = tlidx[g];
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < XP; p++) xs[tid + p * WG] = xv[p];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < UP; q++) {
+            int t = abase + 2 * (tid + q * WG) - base;  // chunk position of the pair's first entry (may be -1)
+            V a0 = (V)a[q].x, a1 = (V)a[q].y;
+            if (OP == SpOp::AXPBY) {
+                a0 = alpha * a0;
+                a1 = alpha * a1;
+            }
+            if (t >= 0 && t < cnt) prod[t] = a0 * xs[ll[q].x];
+            if (t + 1 >= 0 && t + 1 < cnt) prod[t + 1] = a1 * xs[ll[q].y];
+        }
+    }
+    if (tid < RB) rp[tid] = myrp;
+    if (tid == 0) rp[RB] = base + cnt;
+    __syncthreads();
+    int lr = tid / TPR, sub = tid % TPR;
+    int r = r0 + lr;
+    if (r >= n) return;  // whole TPR-lane group
+    int s = rp[lr], e = rp[lr + 1];
+    V acc = 0;
+    if (indexed) {
+        for (int q = s + sub; q < e; q += TPR) acc += prod[q - base];
+    } else {
+        for (int q = s + sub; q < e; q += TPR) {
+            V av = (V)val[q];
+            if (OP == SpOp::AXPBY) av = alpha * av;
+            acc += av * x[col[q]];
+        }
+    }
+#pragma unroll
+    for (int o = TPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, TPR);
+    if (sub == 0) {
+        if (OP == SpOp::AX) y[r] = acc;
+        else if (OP == SpOp::RES) y[r] = b[r] - acc;
+        else if (OP == SpOp::AXPBY) y[r] = acc + beta * y[r];
+        else y[r] = ((-acc + b[r]) / D[r]) + x[r];
+    }
+}
+
 int default_lanes(long long nnz, int n) {
     if (n <= 0) return 1;
     double d = (double)nnz / n;
@@ -721,7 +830,8 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
     const V* D = (const V*)a.D;
     V al = (V)a.alpha, be = (V)a.beta;
     // L < 0: CSR-stream kernels, L = -(1000 s + P + T): T lanes per row
-    // reduction, P = 200 for the (val, col) pair stream, s = 0 for per-XCD
+    // reduction, P = 200 for the (val, col) pair stream, P = 400 for the
+    // row-tile kernel (T = 4, A's tile index in a), s = 0 for per-XCD
     // row segments or s >= 1 for XCD-interleaved super-blocks of 2^(s-1) blocks
     int sb = 0;
     if (L < 0) {
@@ -729,6 +839,17 @@ static void launch1(Context* c, const SpmvArgs& a, int L) {
         if (s > 13) throw Error(BCMGX_ERR_ARG, "stream super-block code out of range");
         sb = s > 0 ? 1 << (s - 1) : 0;
         L = -(-L % 1000);
+    }
+    if (L <= -400) {  // the tiled CSR-stream kernel (TILE_ROWS rows per workgroup), when A's row-tile index is there
+        bool ok = a.tu && a.tucol && a.tlidx && (((uintptr_t)a.val) & (2 * sizeof(T) - 1)) == 0 &&
+                  (((uintptr_t)a.tlidx) & 3) == 0;
+        if (ok) {
+            k_spmv_tiled<T, V, OP><<<grid_for(a.n, TILE_ROWS), WG, 0, c->stream>>>(
+                a.n, a.nnz, a.row, a.col, val, a.tu, a.tucol, a.tlidx, x, y, b, D, al, be, a.stop, sb);
+            BCMGX_LAUNCHED();
+            return;
+        }
+        L = -200 - WG / TILE_ROWS;  // else the paired-load stream kernel with the same lanes per row
     }
     if (L <= -200) {  // CSR-stream variant streaming (val, col) pairs, TPR = -L - 200 lanes per row
         bool aligned = (((uintptr_t)a.val) & (2 * sizeof(T) - 1)) == 0 && (((uintptr_t)a.col) & 7) == 0;

@@ -105,6 +105,7 @@ struct SstepRun {
     void ax(const double* x, double* y, SpOp op = SpOp::AX, const double* b = nullptr) {
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
+        set_tiles(a, A);
         a.x = x; a.y = y; a.b = b; a.k = 1; a.lanes = lanes;
         spmv<double, double>(c, op, a);
     }

@@ -258,7 +258,12 @@ int bcmgx_dump_info(const char* path, const bcmgx_params* p, bcmgx_prec prec, co
  * y = alpha*A*x + beta*y, op as _CSR_vector_mul_mini_warp<OP> (CSR.cu:452-492):
  * op 1: y = A x; op 2: y = y - A x; op 0: y = alpha A x + beta y.
  * lanes: lanes per row (0 = autotuned choice; -T: CSR-stream kernel with T lanes
- * per row reduction; -200-T: the same streaming (val, col) pairs). */
+ * per row reduction; -200-T: the same streaming (val, col) pairs; -404: the
+ * row-tile kernel, 64 rows per workgroup with the x entries of each tile's
+ * distinct columns staged once (builds the matrix's row-tile index on first
+ * use; it is kept with the matrix); -1000 s - code: rows dealt to the XCDs in
+ * super-blocks of 2^(s-1) workgroups).  Every lanes choice of the stream
+ * family (-T, -200-T, -404) reduces a row in the same order for a given T. */
 int bcmgx_spmv(bcmgx_context ctx, bcmgx_matrix A, const double* x, double* y, double alpha,
                double beta, int op, int lanes, int mem);
 /* The same on k right-hand sides at once (x, y column-major with leading

@@ -64,7 +64,7 @@ def oparams(p):
 # ------------------------------------------------------------------- SpMV
 @pytest.mark.parametrize("name", ["lap7_12", "lap27_10", "rand"])
 @pytest.mark.parametrize("lanes", [0, 1, 4, 16, 64, -1, -2, -4, -16, -201, -204, -208, -216,
-                                   -1004, -2204, -3002, -1216, -13204])
+                                   -1004, -2204, -3002, -1216, -13204, -404, -5404])
 def test_spmv_vs_oracle(ctx, name, lanes):
     oa = CASES[name]()
     A = dev(ctx, oa)
@@ -75,6 +75,39 @@ def test_spmv_vs_oracle(ctx, name, lanes):
         ref = O.spmv(oa, x, y, al, be, op)
         scale = np.abs(O.spmv(O.Csr.from_scipy(abs(oa.to_scipy())), np.abs(x))) + np.abs(y)
         assert np.all(np.abs(got - ref) <= 1e-14 * scale + 1e-300), (op, lanes)
+
+
+TILE_CASES = {
+    # 7 entries a row: tiles start at odd positions (the paired loads' first entry belongs to the previous tile)
+    "lap7_18": lambda: O.laplacian3d(7, 18, 18, 18),
+    "lap27_12": lambda: O.laplacian3d(27, 12, 12, 12),
+    # 120 entries a row: every tile too long for the index (read by column index)
+    "long_rows": lambda: O.Csr.from_scipy(rand_spd(300, 0.4, 7)),
+    # about 24 random columns a row over 20000: more distinct columns than a tile indexes
+    "scattered": lambda: O.Csr.from_scipy(rand_spd(20000, 0.0006, 9)),
+    # indexed and unindexed tiles side by side, empty rows included
+    "mixed": lambda: O.Csr.from_scipy(sp.vstack([sp.random(200, 900, density=0.01, random_state=10),
+                                                 sp.random(100, 900, density=0.5, random_state=11),
+                                                 sp.csr_matrix((30, 900)),
+                                                 sp.random(170, 900, density=0.02, random_state=12)]).tocsr()),
+}
+
+
+@pytest.mark.parametrize("name", list(TILE_CASES))
+def test_tiled_spmv_equals_stream_bits(ctx, name):
+    """The row-tile kernel forms the same products and reduces each row in the same
+    order as the paired-load stream kernel with 4 lanes a row: equal bits for every
+    op, whether a tile is indexed or read by column index."""
+    oa = TILE_CASES[name]()
+    A = dev(ctx, oa)
+    rng = np.random.default_rng(17)
+    x, y = rng.normal(size=A.m), rng.normal(size=A.n)
+    for op, al, be in [(1, 1.0, 0.0), (2, -1.0, 1.0), (0, 0.7, -1.3)]:
+        ref = A.spmv(x, y, al, be, op, -204)
+        for lanes in (-404, -5404, -7404):
+            got = A.spmv(x, y, al, be, op, lanes)
+            assert np.array_equal(got.view(np.int64), ref.view(np.int64)), (op, lanes)
+    A.close()
 
 
 SPMM_CASES = {
