@@ -59,6 +59,7 @@ def lib():
         "bcmgx_csr_info": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "bcmgx_csr_copy_to_host": (C.c_int, [_VP, _I32P, _I32P, _F64P]),
         "bcmgx_csr_update_values": (C.c_int, [_VP, _VP, C.c_int]),
+        "bcmgx_csr_analyse": (C.c_int, [_VP, C.POINTER(C.c_double)]),
         "bcmgx_gen_laplacian3d": (C.c_int, [C.c_int] * 7 + [C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                              C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_int)),
                                                              C.POINTER(C.POINTER(C.c_double))]),
@@ -297,6 +298,12 @@ class CSR:
         if val.shape != (self.nnz,):
             raise ValueError(f"expected {self.nnz} values, got shape {val.shape}")
         _check(lib().bcmgx_csr_update_values(self.h, _ptr(val), MEM_HOST), self.ctx)
+
+    def analyse(self):
+        """Build the row-tile index of the pattern now; returns the seconds it took."""
+        t = C.c_double()
+        _check(lib().bcmgx_csr_analyse(self.h, C.byref(t)), self.ctx)
+        return t.value
 
     def to_host(self):
         row = np.zeros(self.n + 1, np.int32)

@@ -252,6 +252,21 @@ int bcmgx_csr_info(bcmgx_matrix A, int* n, int* m, int* nnz) {
     return BCMGX_OK;
 }
 
+int bcmgx_csr_analyse(bcmgx_matrix A, double* seconds) {
+    if (!A) return BCMGX_ERR_ARG;
+    if (!A->ctx) return BCMGX_ERR_STATE;
+    API_BEGIN
+    Context* c = A->ctx->c;
+    BCMGX_HIP(hipSetDevice(c->device));
+    c->sync();
+    double t0 = now_s();
+    build_tile_index(c, A->A);
+    c->sync();
+    if (seconds) *seconds = now_s() - t0;
+    return BCMGX_OK;
+    API_END(A->ctx)
+}
+
 int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem) {
     if (!A || (A->A.nnz > 0 && !val)) return BCMGX_ERR_ARG;
     if (!A->ctx) return BCMGX_ERR_STATE;

@@ -146,6 +146,15 @@ int bcmgx_csr_copy_to_host(bcmgx_matrix A, int* row, int* col, double* val);
  * level-0 product A P of an FP32 hierarchy with post_galerkin; the coarse
  * levels, the prolongators and an AFSAI factor are not rebuilt. */
 int bcmgx_csr_update_values(bcmgx_matrix A, const double* val, int mem);
+/* Build A's row-tile index now (it is built on first need otherwise) and
+ * report the seconds it took.  No counterpart in the reference (its SpMV reads
+ * CSR only).  For every tile of 64 rows the index lists the distinct columns
+ * its entries touch and gives each entry the 16-bit position of its column in
+ * that list; one-column products with A then stage those x entries in LDS once
+ * per tile.  It depends on the pattern only, so it is kept across value
+ * updates and preconditioner setups on A; it costs 2 bytes per entry plus 4
+ * per listed column of device memory. */
+int bcmgx_csr_analyse(bcmgx_matrix A, double* seconds);
 
 /* Host-side problem generators: src/generator/laplacian.cu:119-334 (7-point),
  * :336-777 (27-point), block ordering internal_index (:14-25).  Output arrays
