@@ -208,6 +208,16 @@ int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, c
             BCMGX_HIP(hipMemcpyAsync(h->A.col.get(), col, sizeof(int) * (size_t)nnz, k, c->stream));
             BCMGX_HIP(hipMemcpyAsync(h->A.val.get(), val, sizeof(double) * (size_t)nnz, k, c->stream));
         }
+        if (mem == BCMGX_MEM_HOST) {  // shape facts the kernels use: the longest row, one entry per row
+            int mx = 0;
+            for (int i = 0; i < n; i++) mx = std::max(mx, row[i + 1] - row[i]);
+            h->A.maxrow = mx;
+            if (nnz == n && n > 0) {
+                bool unit = true;
+                for (int i = 0; i <= n && unit; i++) unit = row[i] == i;
+                h->A.unit_rows = unit;
+            }
+        }
         c->sync();
     } catch (...) {
         delete h;

@@ -192,9 +192,13 @@ struct DCsr {
     DBuf<float> valf;  // FP32 copy for the mixed-precision cycle (may be empty)
     DBuf<int> tu, tucol;            // row-tile index (empty until built)
     DBuf<unsigned short> tlidx;
+    bool unit_rows = false;         // row i holds exactly entry i (row pointers 0, 1, .., n): a prolongator
+    int maxrow = -1;                // entries in the longest row (-1: not known)
     bool tiled() const { return (bool)tu; }
     void alloc(Context* c, int n_, int m_, int nnz_) {
         n = n_; m = m_; nnz = nnz_;
+        unit_rows = false;
+        maxrow = -1;
         row.reset(c, (size_t)n + 1);
         col.reset(c, (size_t)(nnz > 0 ? nnz : 1));
         val.reset(c, (size_t)(nnz > 0 ? nnz : 1));

@@ -690,6 +690,8 @@ __global__ void k_iota(int n, int* __restrict__ r) {
 }
 void prolongator_csr(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& P) {
     P.alloc(c, n, nc, n);
+    P.unit_rows = true;
+    P.maxrow = 1;
     k_iota<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, P.row.get());
     BCMGX_LAUNCHED();
     copy_d2d(c, P.col.get(), agg, sizeof(int) * n);
@@ -772,6 +774,31 @@ void transpose_prolongator(Context* c, int n, int nc, const int* agg, const doub
 //   larger             : one workgroup per row, sort in global scratch.
 constexpr int SMALL_P = 64;
 constexpr int MED_P = 2048;  // 2048 x (8 + 8 + 4) B + scan = 41 KiB of LDS
+
+// the largest of n ints into *mx (which holds 0 or a smaller value): a grid-stride
+// pass with one atomic per block, since device-scope atomics on one address serialise
+constexpr int MAX_INT_BLOCKS = 512;
+__global__ __launch_bounds__(WG) void k_max_int(int n, const int* __restrict__ v, int* __restrict__ mx) {
+    __shared__ int part[WG / 64];
+    int x = 0;
+    for (int i = blockIdx.x * WG + (int)threadIdx.x; i < n; i += gridDim.x * WG) x = v[i] > x ? v[i] : x;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < WG / 64; w++) x = part[w] > x ? part[w] : x;
+        if (x > 0) atomicMax(mx, x);
+    }
+}
+
+__global__ void k_row_lengths(int n, const int* __restrict__ arow, long long* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = arow[i + 1] - arow[i];
+}
 
 __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __restrict__ arow, const int* __restrict__ acol,
                                                             const int* __restrict__ brow, long long* __restrict__ cnt) {
@@ -868,12 +895,16 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
 // 32-bit key; the sort then moves one 32-bit word per lane and stage, and each
 // lane fetches its product's value by position afterwards (same order, same
 // sums as the 64-bit key with the value carried along).
-template <int W, bool K32>
+// UNITB: B has exactly one entry per row (a prolongator: row k of B is entry k),
+// so each entry of A makes one product, found without B's row pointers.
+template <int W, bool K32, bool UNITB = false>
 __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
                                                      const int* __restrict__ acol, const double* __restrict__ aval,
                                                      const int* __restrict__ brow, const int* __restrict__ bcol,
                                                      const double* __restrict__ bval, const long long* __restrict__ poff,
-                                                     int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq) {
+                                                     int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq,
+                                                     const int* __restrict__ poff32 = nullptr) {
+    // rows == nullptr: the w-th group takes row w; poff32 (if given) holds the product offsets instead of poff
     constexpr int G = WG / W;  // rows per block
     __shared__ int pre_s[G][W + 1];
     __shared__ double v_s[G][W];
@@ -883,21 +914,25 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     int gshift = wl - lane;                // first wave lane of this row's group
     int w = blockIdx.x * G + gid;
     bool active = w < nrows;  // inactive groups still take part in the barriers
-    int i = active ? rows[w] : 0;
+    int i = active ? (rows ? rows[w] : w) : 0;
     int s = active ? arow[i] : 0, alen = active ? arow[i + 1] - s : 0;
     int k = -1, cntl = 0;
     if (lane < alen) {
         k = acol[s + lane];
-        cntl = brow[k + 1] - brow[k];
+        cntl = UNITB ? 1 : brow[k + 1] - brow[k];
     }
-    // inclusive scan of B-row lengths across the group
+    // inclusive scan of B-row lengths across the group (with unit rows: lane + 1, up to alen)
     int x = cntl;
+    if constexpr (UNITB) {
+        x = lane < alen ? lane + 1 : alen;
+    } else {
 #pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-        int y = __shfl_up(x, d, W);
-        if (lane >= d) x += y;
+        for (int d = 1; d < W; d <<= 1) {
+            int y = __shfl_up(x, d, W);
+            if (lane >= d) x += y;
+        }
     }
-    int total = __shfl(x, W - 1, W);
+    int total = UNITB ? alen : __shfl(x, W - 1, W);
     pre_s[gid][lane + 1] = x;
     if (lane == 0) pre_s[gid][0] = 0;
     __syncthreads();
@@ -914,16 +949,20 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     unsigned key32 = ~0u;
     double v = 0.0;
     if (pos < total) {
-        // owner: largest a with pre[a] <= pos
+        // owner: largest a with pre[a] <= pos (with unit rows: pos itself)
         int lo = 0, hi = alen;  // pre[0] = 0 <= pos
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (pre_s[gid][mid] <= pos) lo = mid;
-            else hi = mid;
+        if constexpr (UNITB) {
+            lo = pos;
+        } else {
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (pre_s[gid][mid] <= pos) lo = mid;
+                else hi = mid;
+            }
         }
         int a = lo;
         int ka = acol[s + a];
-        int jb = brow[ka] + (pos - pre_s[gid][a]);
+        int jb = UNITB ? ka : brow[ka] + (pos - pre_s[gid][a]);
         int cc = bcol[jb];
         v = aval[s + a] * bval[jb];
         key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)pos;
@@ -931,14 +970,21 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     }
     if constexpr (K32) {
         if (pos < total) p_s[gid][pos] = v;
-        bool merge_only = false;
+        bool merge_only = false, sorted = false;
+        unsigned prv = (unsigned)__shfl_up((int)key32, 1, W);
         if (two) {  // both runs in order (run 2 descending over the lanes)?
-            unsigned prv = (unsigned)__shfl_up((int)key32, 1, W);
             bool bad = (lane >= 1 && lane < n1 && !(prv < key32)) || (lane > W - n2 && !(prv > key32));
             unsigned long long bm = (__ballot(bad) >> gshift) & (W == 64 ? ~0ull : ((1ull << W) - 1ull));
             merge_only = bm == 0;
+        } else {
+            // the products already in key order over the lanes (the padding's all-ones keys last):
+            // the case of a prolongator that numbers its aggregates in the order of their rows
+            bool bad = lane >= 1 && lane < total && !(prv < key32);
+            unsigned long long bm = (__ballot(bad) >> gshift) & (W == 64 ? ~0ull : ((1ull << W) - 1ull));
+            sorted = bm == 0;
         }
-        if (merge_only) {
+        if (sorted) {
+        } else if (merge_only) {
             // the last merge of the bitonic sort (all lanes ascending)
 #pragma unroll
             for (int j = W >> 1; j > 0; j >>= 1) {
@@ -996,7 +1042,7 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
             else break;
         }
         int u = __popcll(hm & ((1ull << lane) - 1ull));
-        long long o = poff[i] + u;
+        long long o = (poff32 ? (long long)poff32[i] : poff[i]) + u;
         tcol[o] = colv;
         tval[o] = acc;
     }
@@ -1131,7 +1177,8 @@ __global__ __launch_bounds__(WG) void k_spgemm_large(int nrows, const int* __res
                   gpre + aoff[r], lscan);
 }
 
-__global__ void k_spgemm_compact(int n, const long long* __restrict__ poff, const int* __restrict__ crow,
+template <typename O>
+__global__ void k_spgemm_compact(int n, const O* __restrict__ poff, const int* __restrict__ crow,
                                  const int* __restrict__ tcol, const double* __restrict__ tval, int* __restrict__ ccol,
                                  double* __restrict__ cval) {
     int i = (blockIdx.x * WG + (int)threadIdx.x) / 16;
@@ -1157,18 +1204,62 @@ __global__ void k_large_sizes(int nl, const int* __restrict__ rows, const long l
     asz[r] = (long long)(arow[i + 1] - arow[i]) + 1;
 }
 
+// C's row pointers from the per-row result sizes, its nnz and largest row read back in
+// one round trip, then the rows moved from their product positions (poff) into place
+template <typename O>
+static void spgemm_finish(Context* c, int n, const int* uniq, const O* poff, const int* tcol, const double* tval, DCsr& C) {
+    C.row.reset(c, (size_t)n + 1);
+    DBuf<int> mx(c, 1);
+    fill<int>(c, mx.get(), 1, 0);
+    k_max_int<<<std::min<unsigned>(grid_for(n, WG), MAX_INT_BLOCKS), WG, 0, c->stream>>>(n, uniq, mx.get());
+    BCMGX_LAUNCHED();
+    scan_exclusive_async(c, uniq, C.row.get(), n);
+    int nnz = 0, maxrow = 0;
+    c->d2h2(&nnz, C.row.get() + n, 1, &maxrow, mx.get(), 1);
+    BCMGX_CHECK(nnz >= 0, BCMGX_ERR_UNSUPPORTED, "spgemm: result has more than 2^31 entries");
+    C.nnz = nnz;
+    C.maxrow = maxrow;
+    C.col.reset(c, (size_t)std::max(nnz, 1));
+    C.val.reset(c, (size_t)std::max(nnz, 1));
+    k_spgemm_compact<O><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff, C.row.get(), tcol, tval, C.col.get(), C.val.get());
+    BCMGX_LAUNCHED();
+}
+
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     BCMGX_CHECK(A.m == B.n, BCMGX_ERR_ARG, "spgemm: inner dimensions differ");
     int n = A.n;
     C.n = n;
     C.m = B.m;
+    C.unit_rows = false;
     if (n == 0) {
         C.alloc(c, 0, B.m, 0);
         fill<int>(c, C.row.get(), 1, 0);
         return;
     }
+    bool unitb = B.unit_rows && B.nnz == B.n;  // a prolongator: one product per entry of A
+    // (column, position < 64) packs into 32 bits below the all-ones key of the idle lanes
+    bool k32 = B.m < (1 << 26);
+    if (unitb && k32 && A.maxrow >= 0 && A.maxrow <= 32) {
+        // every row of A fits one group: the products of row i sit at A's own entry positions,
+        // one launch over all rows, no counting, binning or read-back before the numeric pass
+        size_t tsz = (size_t)std::max(A.nnz, 1);
+        DBuf<int> tcol(c, tsz), uniq(c, (size_t)n + 1);
+        DBuf<double> tval(c, tsz);
+        if (A.maxrow <= 16)
+            k_spgemm_small<16, true, true><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(
+                n, nullptr, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(), B.val.get(), nullptr, tcol.get(),
+                tval.get(), uniq.get(), A.row.get());
+        else
+            k_spgemm_small<32, true, true><<<grid_for(n, WG / 32), WG, 0, c->stream>>>(
+                n, nullptr, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(), B.val.get(), nullptr, tcol.get(),
+                tval.get(), uniq.get(), A.row.get());
+        BCMGX_LAUNCHED();
+        spgemm_finish<int>(c, n, uniq.get(), A.row.get(), tcol.get(), tval.get(), C);
+        return;
+    }
     DBuf<long long> cnt(c, n), poff(c, (size_t)n + 1);
-    k_count_products<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
+    if (unitb) k_row_lengths<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), cnt.get());
+    else k_count_products<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
     BCMGX_LAUNCHED();
     scan_exclusive_async(c, cnt.get(), poff.get(), n);
     DBuf<int> bins(c, (size_t)NBINS * n), bincount(c, NBINS);
@@ -1182,14 +1273,12 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     DBuf<int> tcol(c, tsz);
     DBuf<double> tval(c, tsz);
     DBuf<int> uniq(c, (size_t)n + 1);
-    // (column, position < 64) packs into 32 bits below the all-ones key of the idle lanes
-    bool k32 = B.m < (1 << 26);
 #define BCMGX_SMALL(BIN, W)                                                                                                 \
     if (bc[BIN] > 0) {                                                                                                      \
-        auto kern = k32 ? k_spgemm_small<W, true> : k_spgemm_small<W, false>;                                               \
+        auto kern = k32 ? (unitb ? k_spgemm_small<W, true, true> : k_spgemm_small<W, true>) : k_spgemm_small<W, false>;   \
         kern<<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(                                                              \
             bc[BIN], bins.get() + (size_t)(BIN) * n, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(),      \
-            B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get());                                                   \
+            B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get(), nullptr);                                          \
         BCMGX_LAUNCHED();                                                                                                   \
     }
     BCMGX_SMALL(0, 16)
@@ -1219,15 +1308,7 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
         // the scratch goes back to the pool here; the pool hands blocks out again only to
         // work queued behind this kernel on the same stream
     }
-    C.row.reset(c, (size_t)n + 1);
-    long long nnz = scan_exclusive(c, uniq.get(), C.row.get(), n);
-    BCMGX_CHECK(nnz < (1ll << 31), BCMGX_ERR_UNSUPPORTED, "spgemm: result has more than 2^31 entries");
-    C.nnz = (int)nnz;
-    C.col.reset(c, (size_t)std::max<long long>(nnz, 1));
-    C.val.reset(c, (size_t)std::max<long long>(nnz, 1));
-    k_spgemm_compact<<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff.get(), C.row.get(), tcol.get(), tval.get(), C.col.get(),
-                                                                C.val.get());
-    BCMGX_LAUNCHED();
+    spgemm_finish<long long>(c, n, uniq.get(), poff.get(), tcol.get(), tval.get(), C);
 }
 
 // ------------------------------------------------------- row-tile index ---

@@ -251,6 +251,37 @@ def test_spgemm_medium_and_large_rows(ctx):
     _assert_same_csr(dev(ctx, E).spgemm(dev(ctx, E)), O.spgemm(E, E))
 
 
+def test_spgemm_unit_row_right_operand(ctx):
+    """A prolongator-shaped right operand (row k holds entry k) takes the one-product-
+    per-entry path: aggregates numbered out of row order (the products then need the
+    sort), in row order (they arrive sorted), rows of A of 0 to 120 entries (the
+    16-, 32- and 64-lane groups and the LDS bin), equal to the oracle bit for bit.
+    Left operands whose longest row has at most 16 or 32 entries take the single
+    launch over all rows at A's own entry positions; the product of that launch is
+    itself a left operand of the next (its longest row is measured on the device)."""
+    rng = np.random.default_rng(21)
+    n, nc = 3000, 700
+    for top in (30, 16, 32):
+        lens = rng.integers(0, top + 1, n)
+        if top == 30:
+            lens[::97] = 120
+        lens[::53] = 0
+        lens[1::53] = 1
+        lens[2::53] = 2
+        arow = np.concatenate([[0], np.cumsum(lens)])
+        acol = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+        oA = O.Csr.from_arrays(n, n, arow, acol, rng.normal(size=len(acol)))
+        A = dev(ctx, oA)
+        for agg in (rng.integers(0, nc, n), np.sort(rng.integers(0, nc, n))):
+            P = O.Csr.from_arrays(n, nc, np.arange(n + 1), agg.astype(np.int32), rng.normal(size=n))
+            C = A.spgemm(dev(ctx, P))
+            oC = O.spgemm(oA, P)
+            _assert_same_csr(C, oC)
+            # the product (n x nc) times a prolongator of its columns
+            Q = O.Csr.from_arrays(nc, 50, np.arange(nc + 1), rng.integers(0, 50, nc).astype(np.int32), rng.normal(size=nc))
+            _assert_same_csr(C.spgemm(dev(ctx, Q)), O.spgemm(oC, Q))
+
+
 def test_spgemm_two_entry_rows_merge(ctx):
     """Left-operand rows with at most two entries (the restriction's rows) are
     merged from two sorted B rows; both orders of k1 < k2 and k1 == k2, rows of
