@@ -129,8 +129,6 @@ void l1_diag(Context* c, int n, const int* row, const int* col, const double* va
 }
 
 // ------------------------------------------------------ matching weights ---
-constexpr double SUITOR_EPS = 3.885780586188048e-10;  // suitor.h:8
-
 // _makeC (matching.cu:113-135): C_i = a_ii w_i w_i
 __global__ void k_make_C(int n, const double* __restrict__ val, const int* __restrict__ dind, const double* __restrict__ w,
                          double* __restrict__ C) {
@@ -139,73 +137,41 @@ __global__ void k_make_C(int n, const double* __restrict__ val, const int* __res
     C[i] = val[dind[i]] * w[i] * w[i];
 }
 
-// _makeAH_warp (matching.cu:66-111): off-diagonal AH_ij = 1 - 2 a_ij w_i w_j / (C_i + C_j),
-// stored on A's pattern without the diagonal (row i of W starts at row[i] - i).
-// The block also writes its min |AH| to bmin[blockIdx.x], for find_Max_Min
-// (op 1, matching.cu:188-217; a min is order independent).
-template <int L>
-__global__ __launch_bounds__(WG) void k_make_AH(int n, const int* __restrict__ row, const int* __restrict__ col,
-                                                const double* __restrict__ val, const int* __restrict__ dind,
-                                                const double* __restrict__ w, const double* __restrict__ C,
-                                                int* __restrict__ Wcol, double* __restrict__ Wval,
-                                                double* __restrict__ bmin) {
-    __shared__ double sm[WG];
-    int i = (blockIdx.x * WG + (int)threadIdx.x) / L;
-    int lane = (int)threadIdx.x % L;
-    double mn = DBL_MAX;
-    if (i < n) {
-        int s = row[i], e = row[i + 1], d = dind[i];
-        // the row's own C_i, w_i once; C_lo + C_hi and w_lo w_hi in either
-        // operand order are the same bits (IEEE + and * commute)
-        double Ci = C[i], wi = w[i];
-        for (int j = s + lane; j < e; j += L) {
-            if (j == d) continue;
-            int cc = col[j];
-            int o = j - i - (j > d ? 1 : 0);
-            double norm = C[cc] + Ci;
-            double v;
-            if (norm > DBL_EPSILON) {
-                double wt = w[cc] * wi;
-                v = 1. - ((2. * val[j] * wt) / norm);
-            } else {
-                v = DBL_EPSILON;
-            }
-            Wcol[o] = cc;
-            Wval[o] = v;
-            mn = fmin(mn, fabs(v));
-        }
-    }
-    // the min over each wave by shuffles, then over the block's waves
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mn;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = sm[0];
-        for (int q = 1; q < WG / 64; q++) b = fmin(b, sm[q]);
-        bmin[blockIdx.x] = b;
-    }
+// The matching graph W in one pass over A: for every off-diagonal entry, AH
+// (_makeAH_warp, matching.cu:66-111), its weight (_make_w, :219-229) and the
+// symmetrisation W(i, c) <- W(c, i) for c < i (_write_T_warp, :29-64); W lies
+// on A's pattern without the diagonal (row i of W starts at row[i] - i).
+//
+//   AH_ij = 1 - 2 a_ij w_i w_j / (C_i + C_j)   (DBL_EPSILON when C_i + C_j <= DBL_EPSILON)
+//   W_ij  = log |AH_ij|, or log(SUITOR_EPS / (0.999 m)) when AH_ij = 0,
+//
+// where m is the least |AH| over all entries, clamped up to DBL_EPSILON
+// (toMaximumProductMatrix, :231-250).  m only enters the weight of an entry
+// whose AH is 0, and then m = 0 itself, so that weight is always
+// log(SUITOR_EPS / (0.999 DBL_EPSILON)): no pass over AH for its minimum.
+//
+// A lower entry (c < i) takes the weight of the mirrored entry (c, i), whose AH
+// is computed here from a_ci: C_i + C_c and w_i w_c are the same bits in either
+// operand order, so it equals what row c computes for that entry.  A mirrored
+// pair of positive weight -- only such a pair can carry a proposal
+// (find_Candidate, suitor.cu:91-94) -- is recorded both ways in mir (filled
+// with -1 beforehand: other entries keep it).
+//
+// Preference ranks.  Vertex i orders its neighbours c by (W(i,c), c) -- the
+// Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
+// rank[e] = position of entry e of row i in that order (0: the least liked).
+// A row of at most L off-diagonal entries keeps one entry per lane, staged in
+// LDS, where the group reads every entry of the row by broadcast; a longer row
+// reads its weights back from W after the block's barrier (the group wrote them).
+constexpr double SUITOR_EPS = 3.885780586188048e-10;  // suitor.h:8
+
+__device__ __forceinline__ double ah_of(double a, double wt, double norm) {
+    return norm > DBL_EPSILON ? 1. - ((2. * a * wt) / norm) : DBL_EPSILON;
 }
 
-// min over the blocks' minima into *out: non-negative doubles order like their bit patterns
-__global__ __launch_bounds__(WG) void k_absmin(long long m, const double* __restrict__ v, unsigned long long* __restrict__ out) {
-    __shared__ double s[WG];
-    double mn = DBL_MAX;
-    for (long long t = (long long)blockIdx.x * WG + threadIdx.x; t < m; t += (long long)gridDim.x * WG) mn = fmin(mn, v[t]);
-    s[threadIdx.x] = mn;
-    __syncthreads();
-    for (int st = WG / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + st]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMin(out, (unsigned long long)__double_as_longlong(s[0]));
-}
-
-// _make_w (matching.cu:219-229) with toMaximumProductMatrix's clamp (:241-245)
-__device__ __forceinline__ double weight_of(double v, double mn) {
-    if (fabs(mn) < DBL_EPSILON) mn = DBL_EPSILON;
+__device__ __forceinline__ double weight_of(double v) {
     double s = fabs(v);
-    return log(s != 0.0 ? s : SUITOR_EPS / (0.999 * mn));
+    return log(s != 0.0 ? s : SUITOR_EPS / (0.999 * DBL_EPSILON));
 }
 
 __device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
@@ -218,69 +184,65 @@ __device__ __forceinline__ int lower_bound_dev(const int* a, int len, int key) {
     return lo;
 }
 
-// _make_w (matching.cu:219-229) then _write_T_warp (matching.cu:29-64),
-// W(i, c) <- W(c, i) for c < i, in one pass from AH into a new array W, and
-// each row's preference ranks.  A mirrored pair (i, c) / (c, i) has one weight
-// for both entries; when that weight is positive -- only then can the pair
-// carry a proposal (find_Candidate, suitor.cu:91-94) -- the pair is recorded
-// both ways in mir (filled with -1 beforehand: other entries keep it).
-//
-// Preference ranks.  Vertex i orders its neighbours c by (W(i,c), c) -- the
-// Suitor tie rule "heavier, then higher id" (suitor.cu:108,115,136-140).
-// rank[e] = position of entry e of row i in that order (0: the least liked).
-// A row of at most L entries keeps one entry per lane, staged in LDS, where
-// the group reads every entry of the row by broadcast; a longer row reads its
-// weights back from W after the block's barrier (the group wrote them).
 template <int L>
-__global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict__ Wrow, const int* __restrict__ Wcol,
-                                                   const double* __restrict__ AH, double* W, int* __restrict__ mir,
-                                                   int* __restrict__ rank, const unsigned long long* __restrict__ minbits) {
+__global__ __launch_bounds__(WG) void k_weights(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                const double* __restrict__ val, const int* __restrict__ dind,
+                                                const double* __restrict__ w, const double* __restrict__ C,
+                                                int* __restrict__ Wcol, double* W, int* __restrict__ mir,
+                                                int* __restrict__ rank) {
     __shared__ double sw[WG];
     __shared__ int sc[WG];
     int tid = (int)threadIdx.x;
     int i = (blockIdx.x * WG + tid) / L;
     int lane = tid % L;
     int gb = tid - lane;
-    double mn = __longlong_as_double((long long)*minbits);
-    int si = 0, ei = 0;
+    int s = 0, e = 1, d = 0;
+    double Ci = 0.0, wi = 0.0;
     if (i < n) {
-        si = Wrow[i];
-        ei = Wrow[i + 1];
+        s = row[i];
+        e = row[i + 1];
+        d = dind[i];
+        Ci = C[i];
+        wi = w[i];
     }
-    int len = ei - si;
-    double wa = 0.0;  // this lane's first entry (the only one when len <= L)
+    int si = s - i;     // W's row start
+    int len = e - s - 1;  // off-diagonal entries
+    double wa = 0.0;    // this lane's first entry (the only one when len <= L)
     int ca = 0;
-    for (int j = si + lane; j < ei; j += L) {
-        int cc = Wcol[j];
-        int src = j;
-        double ah;
+    for (int t = lane; t < len; t += L) {
+        int j = s + t + (s + t >= d ? 1 : 0);  // A's position of the t-th off-diagonal entry
+        int o = si + t;
+        int cc = col[j];
+        double a = val[j];
+        int src = o;
         if (cc < i) {
-            int b = Wrow[cc], lc = Wrow[cc + 1] - b;
+            int b = row[cc], lc = row[cc + 1] - b;
             // a symmetric stencil puts the mirror at the reflected position of a
-            // row of the same length: one probe, whose AH value is read with it
-            // (b + p < Wrow[cc + 1] <= j: in range), else a binary search
-            int p = lc - 1 - (j - si);
-            bool inr = lc == len && p >= 0 && p < lc;
+            // row of the same length: one probe, whose value is read with it
+            // (b + p < row[cc + 1] <= j: in range), else a binary search
+            int p = lc - 1 - (j - s);
+            bool inr = lc == e - s && p >= 0 && p < lc;
             int q = inr ? b + p : j;
-            int wq = Wcol[q];
-            ah = AH[q];
-            if (inr && wq == i) {
-                src = q;  // mirrored entry present
-            } else {
-                p = lower_bound_dev(Wcol + b, lc, i);
-                if (p < lc && Wcol[b + p] == i) src = b + p;
-                ah = AH[src];
+            int cq = col[q];
+            double aq = val[q];
+            if (!(inr && cq == i)) {
+                p = lower_bound_dev(col + b, lc, i);
+                q = p < lc && col[b + p] == i ? b + p : -1;
+                aq = q >= 0 ? val[q] : a;
             }
-        } else {
-            ah = AH[j];
+            if (q >= 0) {
+                a = aq;
+                src = q - cc - (q > dind[cc] ? 1 : 0);  // the mirror's place in W
+            }
         }
-        double wv = weight_of(ah, mn);
-        W[j] = wv;
-        if (src != j && wv > 0) {
-            mir[j] = src;
-            mir[src] = j;
+        double wv = weight_of(ah_of(a, w[cc] * wi, C[cc] + Ci));
+        Wcol[o] = cc;
+        W[o] = wv;
+        if (src != o && wv > 0) {
+            mir[o] = src;
+            mir[src] = o;
         }
-        if (j == si + lane) {
+        if (t == lane) {
             wa = wv;
             ca = cc;
         }
@@ -301,13 +263,13 @@ __global__ __launch_bounds__(WG) void k_symmetrize(int n, const int* __restrict_
         }
         return;
     }
-    for (int a = si + lane; a < ei; a += L) {
-        double w = W[a];
+    for (int a = si + lane; a < si + len; a += L) {
+        double wv = W[a];
         int cw = Wcol[a];
         int r = 0;
-        for (int b = si; b < ei; b++) {
+        for (int b = si; b < si + len; b++) {
             double wb = W[b];
-            r += (wb < w || (wb == w && Wcol[b] < cw)) ? 1 : 0;
+            r += (wb < wv || (wb == wv && Wcol[b] < cw)) ? 1 : 0;
         }
         rank[a] = r;
     }
@@ -528,38 +490,20 @@ void suitor_match(Context* c, const DCsr& A, const double* w, int* M, const int*
     k_make_C<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.val.get(), dind, w, C.get());
     BCMGX_LAUNCHED();
     DBuf<int> Wrow(c, (size_t)n + 1), Wcol(c, mm);
-    DBuf<double> Wval(c, mm);
     k_make_wrow<<<grid_for(n + 1, WG), WG, 0, c->stream>>>(n, A.row.get(), Wrow.get());
     BCMGX_LAUNCHED();
     int L = lanes_for(A.nnz, n);
-    DBuf<unsigned long long> mn(c, 1);
-    double big = DBL_MAX;
-    long long bigbits;
-    std::memcpy(&bigbits, &big, sizeof(big));
-    fill<long long>(c, (long long*)mn.get(), 1, bigbits);
-    unsigned nb_ah = grid_for(n, WG / L);
-    DBuf<double> bmin(c, nb_ah);
-#define BCMGX_AH(LL)                                                                                               \
-    case LL:                                                                                                       \
-        k_make_AH<LL><<<nb_ah, WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(), dind, w,            \
-                                                   C.get(), Wcol.get(), Wval.get(), bmin.get());                   \
-        break;
-    switch (L) { BCMGX_AH(4) BCMGX_AH(8) BCMGX_AH(16) BCMGX_AH(32) }
-#undef BCMGX_AH
-    BCMGX_LAUNCHED();
-    k_absmin<<<(unsigned)std::min<long long>(grid_for(nb_ah, WG), 256), WG, 0, c->stream>>>(nb_ah, bmin.get(), mn.get());
-    BCMGX_LAUNCHED();
     DBuf<int> mir(c, mm);
     fill<int>(c, mir.get(), (long long)mm, -1);
-    DBuf<double> Wv(c, mm);  // the weights, from AH (Wval)
+    DBuf<double> Wv(c, mm);  // the weights
     DBuf<int> rank(c, mm);
-#define BCMGX_SY(LL)                                                                                              \
-    case LL:                                                                                                      \
-        k_symmetrize<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, Wrow.get(), Wcol.get(), Wval.get(), Wv.get(), \
-                                                                     mir.get(), rank.get(), mn.get());            \
+#define BCMGX_W(LL)                                                                                                   \
+    case LL:                                                                                                          \
+        k_weights<LL><<<grid_for(n, WG / LL), WG, 0, c->stream>>>(n, A.row.get(), A.col.get(), A.val.get(), dind, w,   \
+                                                                  C.get(), Wcol.get(), Wv.get(), mir.get(), rank.get()); \
         break;
-    switch (L) { BCMGX_SY(4) BCMGX_SY(8) BCMGX_SY(16) BCMGX_SY(32) }
-#undef BCMGX_SY
+    switch (L) { BCMGX_W(4) BCMGX_W(8) BCMGX_W(16) BCMGX_W(32) }
+#undef BCMGX_W
     BCMGX_LAUNCHED();
     DBuf<int2> pref(c, mm);
     DBuf<int> pos(c, n);

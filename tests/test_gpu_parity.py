@@ -147,6 +147,41 @@ def test_matching_bit_exact(ctx, name):
     assert np.array_equal(A.matching(w), O.suitor(oa, w))
 
 
+def _zero_ah_matrix(n, seed):
+    """Symmetric pattern, diagonal values in {0, 2, 4, 6} (explicit zeros kept), and a
+    third of the off-diagonal pairs set to a_ij = (a_ii + a_jj) / 2, whose AH is
+    exactly 0 for w = 1; pairs of zero diagonals have C_i + C_j = 0."""
+    rng = np.random.default_rng(seed)
+    m = 4 * n
+    i = rng.integers(0, n, m)
+    j = rng.integers(0, n, m)
+    keep = i != j
+    i, j = i[keep], j[keep]
+    pairs = np.unique(np.stack([np.minimum(i, j), np.maximum(i, j)], 1), axis=0)
+    d = rng.choice([0.0, 2.0, 4.0, 6.0], n)
+    v = -rng.uniform(0.1, 3.0, len(pairs))
+    z = rng.random(len(pairs)) < 1 / 3
+    v[z] = (d[pairs[z, 0]] + d[pairs[z, 1]]) / 2
+    r = np.concatenate([pairs[:, 0], pairs[:, 1], np.arange(n)])
+    c = np.concatenate([pairs[:, 1], pairs[:, 0], np.arange(n)])
+    x = np.concatenate([v, v, d])
+    o = np.lexsort((c, r))
+    r, c, x = r[o], c[o], x[o]
+    row = np.searchsorted(r, np.arange(n + 1)).astype(np.int32)
+    return O.Csr.from_arrays(n, n, row, c.astype(np.int32), x)
+
+
+def test_matching_zero_ah_and_zero_diagonals(ctx):
+    """Entries whose AH is exactly 0 weigh log(SUITOR_EPS / (0.999 DBL_EPSILON)), the
+    least |AH| being 0 and clamped; pairs of zero diagonals take AH = DBL_EPSILON;
+    with w = 1 (exact zeros) and a random w (none), equal to the oracle."""
+    for seed in (1, 2):
+        oa = _zero_ah_matrix(700, seed)
+        A = dev(ctx, oa)
+        for w in (np.ones(A.n), np.random.default_rng(seed).uniform(0.5, 2.0, A.n)):
+            assert np.array_equal(A.matching(w), O.suitor(oa, w)), seed
+
+
 @pytest.mark.parametrize("name", ["lap27_40", "lap7_64", "long_rows", "long_rows_ties"])
 def test_matching_chains_and_long_rows(ctx, name):
     """Grids large enough for displacement chains to span several rounds, and
@@ -305,6 +340,59 @@ def test_spgemm_two_entry_rows_merge(ctx):
         arow.append(len(acol))
     oA = O.Csr.from_arrays(na, nb, np.array(arow), np.array(acol, np.int32), rng.normal(size=len(acol)))
     _assert_same_csr(dev(ctx, oA).spgemm(dev(ctx, oB)), O.spgemm(oA, oB))
+
+
+def test_spgemm_two_entry_rows_out_of_order(ctx):
+    """The two-run merge falls back to the full sort when a run is out of order:
+    B rows with shuffled and repeated columns, A rows listing the larger B row
+    first.  Products of one column are summed in the order they are generated
+    (A's entry order, then B's), as the oracle does, so the result is bit-exact."""
+    rng = np.random.default_rng(12)
+    nb, m = 300, 200
+    lens = rng.integers(0, 30, nb)
+    brow = np.concatenate([[0], np.cumsum(lens)])
+    bcol = np.concatenate([rng.integers(0, m, l) for l in lens]).astype(np.int32)  # unsorted, repeats
+    oB = O.Csr.from_arrays(nb, m, brow, bcol, rng.normal(size=len(bcol)))
+    na = 400
+    arow, acol = [0], []
+    for i in range(na):
+        t = i % 4
+        ks = [] if t == 0 else [int(rng.integers(nb))] if t == 1 else sorted(rng.choice(nb, 2, replace=False).tolist())
+        if t == 3:
+            ks = ks[::-1]  # larger B row first
+        acol += ks
+        arow.append(len(acol))
+    oA = O.Csr.from_arrays(na, nb, np.array(arow), np.array(acol, np.int32), rng.normal(size=len(acol)))
+    _assert_same_csr(dev(ctx, oA).spgemm(dev(ctx, oB)), O.spgemm(oA, oB))
+
+
+def test_spgemm_wide_right_operand(ctx):
+    """A right operand with 2^26 or more columns takes the 64-bit (column, position)
+    keys.  The expected product comes from the oracle on B with its columns renumbered
+    densely in the same order (a monotone map leaves every sum's order unchanged),
+    mapped back afterwards."""
+    rng = np.random.default_rng(13)
+    n, m = 2000, (1 << 26) + 7
+    used = np.sort(rng.choice(m, 900, replace=False))
+    used[-1] = m - 1
+    lens = rng.integers(0, 12, n)
+    lens[::41] = 40
+    brow = np.concatenate([[0], np.cumsum(lens)])
+    bloc = np.concatenate([np.sort(rng.choice(len(used), l, replace=False)) for l in lens]).astype(np.int32)
+    bval = rng.normal(size=len(bloc))
+    alens = rng.integers(0, 8, n)
+    arow = np.concatenate([[0], np.cumsum(alens)])
+    acol = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in alens]).astype(np.int32)
+    oA = O.Csr.from_arrays(n, n, arow, acol, rng.normal(size=len(acol)))
+    ref = O.spgemm(oA, O.Csr.from_arrays(n, len(used), brow, bloc, bval))
+    rr, rc, rv = ref.arrays()
+    B = bx.CSR(ctx, n, m, brow, used[bloc].astype(np.int32), bval)
+    C = dev(ctx, oA).spgemm(B)
+    cr, cc, cv = C.to_host()
+    assert (C.n, C.m) == (n, m)
+    np.testing.assert_array_equal(cr, rr)
+    np.testing.assert_array_equal(cc, used[rc])
+    np.testing.assert_array_equal(cv, rv)
 
 
 def test_kernel_timers(ctx):
