@@ -199,11 +199,10 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             suitor_match(c, *Ai, wi.get(), M.get(), Ai == &A0 ? dind0.get() : nullptr);
             int nc = make_prolongator(c, n, M.get(), wi.get(), agg_i.get(), pval_i.get());
             BCMGX_CHECK(nc > 0, BCMGX_ERR_NUMERIC, "coarsening produced an empty level");
-            DCsr Pi, Ri, AP, Anew;
+            DCsr Pi, Ri, Anew;
             prolongator_csr(c, n, nc, agg_i.get(), pval_i.get(), Pi);
             transpose_prolongator(c, n, nc, agg_i.get(), pval_i.get(), Ri);
-            spgemm(c, *Ai, Pi, AP);
-            spgemm(c, Ri, AP, Anew);
+            galerkin(c, *Ai, Pi, Ri, Anew);
             DBuf<double> wn(c, nc);
             {
                 SpmvArgs a;  // w_{i+1} = R_i w_i, one lane per row (sequential sums)

@@ -236,6 +236,8 @@ void transpose_prolongator(Context* c, int n, int nc, const int* agg, const doub
 void transpose(Context* c, const DCsr& A, DCsr& T);
 // C = A B, deterministic ascending-k accumulation, sorted columns
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
+// C = R (A P), the same bits as spgemm(R, spgemm(A, P)); A P is not compacted when P is a prolongator
+void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C);
 // P <- P P_i for one-entry-per-row operators: agg[k] = agg_i[agg[k]], pval[k] *= pval_i[agg[k]]
 void compose_prolongators(Context* c, int n, int* agg, double* pval, const int* agg_i, const double* pval_i);
 // csr of a one-entry-per-row prolongator (row = identity)

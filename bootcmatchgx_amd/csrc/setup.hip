@@ -744,8 +744,11 @@ __global__ void k_row_lengths(int n, const int* __restrict__ arow, long long* __
     if (i < n) cnt[i] = arow[i + 1] - arow[i];
 }
 
+// B's row k is [brow[k], bend[k]) in every product kernel: bend = brow + 1 for a
+// CSR matrix, or the ends of rows kept in place with gaps after them
 __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __restrict__ arow, const int* __restrict__ acol,
-                                                            const int* __restrict__ brow, long long* __restrict__ cnt) {
+                                                            const int* __restrict__ brow, const int* __restrict__ bend,
+                                                            long long* __restrict__ cnt) {
     __shared__ int sc[RS_CAP];
     int r0 = blockIdx.x * RS_ROWS;
     int rend = r0 + RS_ROWS < n ? r0 + RS_ROWS : n;
@@ -764,7 +767,7 @@ __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 int k = sc[j + u - base];
-                d[u] = brow[k + 1] - brow[k];
+                d[u] = bend[k] - brow[k];
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) s += d[u];
@@ -772,7 +775,7 @@ __global__ __launch_bounds__(RS_ROWS) void k_count_products(int n, const int* __
     }
     for (; j < je; j++) {
         int k = staged ? sc[j - base] : acol[j];
-        s += brow[k + 1] - brow[k];
+        s += bend[k] - brow[k];
     }
     cnt[i] = s;
 }
@@ -844,11 +847,14 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
 template <int W, bool K32, bool UNITB = false>
 __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
                                                      const int* __restrict__ acol, const double* __restrict__ aval,
-                                                     const int* __restrict__ brow, const int* __restrict__ bcol,
-                                                     const double* __restrict__ bval, const long long* __restrict__ poff,
-                                                     int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq,
-                                                     const int* __restrict__ poff32 = nullptr) {
-    // rows == nullptr: the w-th group takes row w; poff32 (if given) holds the product offsets instead of poff
+                                                     const int* __restrict__ brow, const int* __restrict__ bend,
+                                                     const int* __restrict__ bcol, const double* __restrict__ bval,
+                                                     const long long* __restrict__ poff, int* __restrict__ tcol,
+                                                     double* __restrict__ tval, int* __restrict__ uniq,
+                                                     const int* __restrict__ poff32, int* __restrict__ rend) {
+    // rows == nullptr: the w-th group takes row w; poff32 (if given) holds the product offsets
+    // instead of poff; uniq (if given) receives each row's result length, rend (if given) its end
+    // in tcol / tval (with poff32)
     constexpr int G = WG / W;  // rows per block
     __shared__ int pre_s[G][W + 1];
     __shared__ double v_s[G][W];
@@ -863,7 +869,7 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     int k = -1, cntl = 0;
     if (lane < alen) {
         k = acol[s + lane];
-        cntl = UNITB ? 1 : brow[k + 1] - brow[k];
+        cntl = UNITB ? 1 : bend[k] - brow[k];
     }
     // inclusive scan of B-row lengths across the group (with unit rows: lane + 1, up to alen)
     int x = cntl;
@@ -990,12 +996,17 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
         tcol[o] = colv;
         tval[o] = acc;
     }
-    if (lane == 0 && active) uniq[i] = __popcll(hm);
+    if (lane == 0 && active) {
+        int u = __popcll(hm);
+        if (uniq) uniq[i] = u;
+        if (rend) rend[i] = poff32[i] + u;
+    }
 }
 
 // one workgroup per row; keys/vals/pre point at LDS (medium) or global scratch (large)
 __device__ void spgemm_row_wg(int i, const int* __restrict__ arow, const int* __restrict__ acol, const double* __restrict__ aval,
-                              const int* __restrict__ brow, const int* __restrict__ bcol, const double* __restrict__ bval,
+                              const int* __restrict__ brow, const int* __restrict__ bend, const int* __restrict__ bcol,
+                              const double* __restrict__ bval,
                               long long o0, int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq,
                               unsigned long long* keys, double* vals, int* pre, int* lscan) {
     int tid = threadIdx.x;
@@ -1007,7 +1018,7 @@ __device__ void spgemm_row_wg(int i, const int* __restrict__ arow, const int* __
         int cl = 0;
         if (a < alen) {
             int k = acol[s + a];
-            cl = brow[k + 1] - brow[k];
+            cl = bend[k] - brow[k];
         }
         lscan[tid] = cl;
         __syncthreads();
@@ -1096,28 +1107,29 @@ __device__ void spgemm_row_wg(int i, const int* __restrict__ arow, const int* __
 
 __global__ __launch_bounds__(WG) void k_spgemm_med(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
                                                    const int* __restrict__ acol, const double* __restrict__ aval,
-                                                   const int* __restrict__ brow, const int* __restrict__ bcol,
-                                                   const double* __restrict__ bval, const long long* __restrict__ poff,
-                                                   int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq) {
+                                                   const int* __restrict__ brow, const int* __restrict__ bend,
+                                                   const int* __restrict__ bcol, const double* __restrict__ bval,
+                                                   const long long* __restrict__ poff, int* __restrict__ tcol,
+                                                   double* __restrict__ tval, int* __restrict__ uniq) {
     __shared__ unsigned long long keys[MED_P];
     __shared__ double vals[MED_P];
     __shared__ int pre[MED_P + 1];
     __shared__ int lscan[WG];
     int i = rows[blockIdx.x];
-    spgemm_row_wg(i, arow, acol, aval, brow, bcol, bval, poff[i], tcol, tval, uniq, keys, vals, pre, lscan);
+    spgemm_row_wg(i, arow, acol, aval, brow, bend, bcol, bval, poff[i], tcol, tval, uniq, keys, vals, pre, lscan);
 }
 
 __global__ __launch_bounds__(WG) void k_spgemm_large(int nrows, const int* __restrict__ rows, const int* __restrict__ arow,
                                                      const int* __restrict__ acol, const double* __restrict__ aval,
-                                                     const int* __restrict__ brow, const int* __restrict__ bcol,
-                                                     const double* __restrict__ bval, const long long* __restrict__ poff,
-                                                     int* __restrict__ tcol, double* __restrict__ tval, int* __restrict__ uniq,
-                                                     unsigned long long* gkeys, double* gvals, int* gpre,
+                                                     const int* __restrict__ brow, const int* __restrict__ bend,
+                                                     const int* __restrict__ bcol, const double* __restrict__ bval,
+                                                     const long long* __restrict__ poff, int* __restrict__ tcol,
+                                                     double* __restrict__ tval, int* __restrict__ uniq, unsigned long long* gkeys, double* gvals, int* gpre,
                                                      const long long* __restrict__ soff, const long long* __restrict__ aoff) {
     __shared__ int lscan[WG];
     int r = blockIdx.x;
     int i = rows[r];
-    spgemm_row_wg(i, arow, acol, aval, brow, bcol, bval, poff[i], tcol, tval, uniq, gkeys + soff[r], gvals + soff[r],
+    spgemm_row_wg(i, arow, acol, aval, brow, bend, bcol, bval, poff[i], tcol, tval, uniq, gkeys + soff[r], gvals + soff[r],
                   gpre + aoff[r], lscan);
 }
 
@@ -1169,41 +1181,24 @@ static void spgemm_finish(Context* c, int n, const int* uniq, const O* poff, con
     BCMGX_LAUNCHED();
 }
 
-void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
-    BCMGX_CHECK(A.m == B.n, BCMGX_ERR_ARG, "spgemm: inner dimensions differ");
+// The right operand of a product: row k is [row[k], end[k]) of col / val
+struct BRows {
+    int n, m;
+    const int* row;
+    const int* end;
+    const int* col;
+    const double* val;
+    bool unit;  // row k holds exactly entry k
+};
+
+// A B by the sort-based kernels (count, bin by product count, numeric, compact)
+static void spgemm_rows(Context* c, const DCsr& A, const BRows& B, DCsr& C) {
     int n = A.n;
-    C.n = n;
-    C.m = B.m;
-    C.unit_rows = false;
-    if (n == 0) {
-        C.alloc(c, 0, B.m, 0);
-        fill<int>(c, C.row.get(), 1, 0);
-        return;
-    }
-    bool unitb = B.unit_rows && B.nnz == B.n;  // a prolongator: one product per entry of A
-    // (column, position < 64) packs into 32 bits below the all-ones key of the idle lanes
+    bool unitb = B.unit;
     bool k32 = B.m < (1 << 26);
-    if (unitb && k32 && A.maxrow >= 0 && A.maxrow <= 32) {
-        // every row of A fits one group: the products of row i sit at A's own entry positions,
-        // one launch over all rows, no counting, binning or read-back before the numeric pass
-        size_t tsz = (size_t)std::max(A.nnz, 1);
-        DBuf<int> tcol(c, tsz), uniq(c, (size_t)n + 1);
-        DBuf<double> tval(c, tsz);
-        if (A.maxrow <= 16)
-            k_spgemm_small<16, true, true><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(
-                n, nullptr, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(), B.val.get(), nullptr, tcol.get(),
-                tval.get(), uniq.get(), A.row.get());
-        else
-            k_spgemm_small<32, true, true><<<grid_for(n, WG / 32), WG, 0, c->stream>>>(
-                n, nullptr, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(), B.val.get(), nullptr, tcol.get(),
-                tval.get(), uniq.get(), A.row.get());
-        BCMGX_LAUNCHED();
-        spgemm_finish<int>(c, n, uniq.get(), A.row.get(), tcol.get(), tval.get(), C);
-        return;
-    }
     DBuf<long long> cnt(c, n), poff(c, (size_t)n + 1);
     if (unitb) k_row_lengths<<<grid_for(n, WG), WG, 0, c->stream>>>(n, A.row.get(), cnt.get());
-    else k_count_products<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row.get(), cnt.get());
+    else k_count_products<<<grid_for(n, RS_ROWS), RS_ROWS, 0, c->stream>>>(n, A.row.get(), A.col.get(), B.row, B.end, cnt.get());
     BCMGX_LAUNCHED();
     scan_exclusive_async(c, cnt.get(), poff.get(), n);
     DBuf<int> bins(c, (size_t)NBINS * n), bincount(c, NBINS);
@@ -1220,9 +1215,10 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
 #define BCMGX_SMALL(BIN, W)                                                                                                 \
     if (bc[BIN] > 0) {                                                                                                      \
         auto kern = k32 ? (unitb ? k_spgemm_small<W, true, true> : k_spgemm_small<W, true>) : k_spgemm_small<W, false>;   \
-        kern<<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(                                                              \
-            bc[BIN], bins.get() + (size_t)(BIN) * n, A.row.get(), A.col.get(), A.val.get(), B.row.get(), B.col.get(),      \
-            B.val.get(), poff.get(), tcol.get(), tval.get(), uniq.get(), nullptr);                                          \
+        kern<<<grid_for(bc[BIN], WG / W), WG, 0, c->stream>>>(bc[BIN], bins.get() + (size_t)(BIN) * n, A.row.get(),       \
+                                                              A.col.get(), A.val.get(), B.row, B.end, B.col, B.val,        \
+                                                              poff.get(), tcol.get(), tval.get(), uniq.get(), nullptr,     \
+                                                              nullptr);                                                    \
         BCMGX_LAUNCHED();                                                                                                   \
     }
     BCMGX_SMALL(0, 16)
@@ -1231,8 +1227,7 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
 #undef BCMGX_SMALL
     if (bc[3] > 0) {
         k_spgemm_med<<<bc[3], WG, 0, c->stream>>>(bc[3], bins.get() + 3 * (size_t)n, A.row.get(), A.col.get(), A.val.get(),
-                                                  B.row.get(), B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(),
-                                                  uniq.get());
+                                                  B.row, B.end, B.col, B.val, poff.get(), tcol.get(), tval.get(), uniq.get());
         BCMGX_LAUNCHED();
     }
     if (bc[4] > 0) {
@@ -1246,13 +1241,77 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
         DBuf<double> gv(c, (size_t)ptot);
         DBuf<int> gp(c, (size_t)atot);
         k_spgemm_large<<<nl, WG, 0, c->stream>>>(nl, bins.get() + 4 * (size_t)n, A.row.get(), A.col.get(), A.val.get(),
-                                                 B.row.get(), B.col.get(), B.val.get(), poff.get(), tcol.get(), tval.get(),
-                                                 uniq.get(), gk.get(), gv.get(), gp.get(), soff.get(), aoff.get());
+                                                 B.row, B.end, B.col, B.val, poff.get(), tcol.get(), tval.get(), uniq.get(),
+                                                 gk.get(), gv.get(), gp.get(), soff.get(), aoff.get());
         BCMGX_LAUNCHED();
         // the scratch goes back to the pool here; the pool hands blocks out again only to
         // work queued behind this kernel on the same stream
     }
     spgemm_finish<long long>(c, n, uniq.get(), poff.get(), tcol.get(), tval.get(), C);
+}
+
+// A P with P a prolongator (one entry per row) and every row of A within one
+// 32-lane group: the products of row i sit at A's own entry positions, one
+// launch over all rows with no counting, binning or read-back.  Row i of the
+// result is left at [A.row[i], rend[i]) of tcol / tval.
+static bool unit_product_fits(const DCsr& A, const DCsr& P) {
+    return P.unit_rows && P.nnz == P.n && P.m < (1 << 26) && A.maxrow >= 0 && A.maxrow <= 32;
+}
+static void unit_product_in_place(Context* c, const DCsr& A, const DCsr& P, int* tcol, double* tval, int* uniq, int* rend) {
+    int n = A.n;
+    if (A.maxrow <= 16)
+        k_spgemm_small<16, true, true><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(
+            n, nullptr, A.row.get(), A.col.get(), A.val.get(), P.row.get(), P.row.get() + 1, P.col.get(), P.val.get(), nullptr,
+            tcol, tval, uniq, A.row.get(), rend);
+    else
+        k_spgemm_small<32, true, true><<<grid_for(n, WG / 32), WG, 0, c->stream>>>(
+            n, nullptr, A.row.get(), A.col.get(), A.val.get(), P.row.get(), P.row.get() + 1, P.col.get(), P.val.get(), nullptr,
+            tcol, tval, uniq, A.row.get(), rend);
+    BCMGX_LAUNCHED();
+}
+
+void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
+    BCMGX_CHECK(A.m == B.n, BCMGX_ERR_ARG, "spgemm: inner dimensions differ");
+    int n = A.n;
+    C.n = n;
+    C.m = B.m;
+    C.unit_rows = false;
+    if (n == 0) {
+        C.alloc(c, 0, B.m, 0);
+        fill<int>(c, C.row.get(), 1, 0);
+        return;
+    }
+    if (unit_product_fits(A, B)) {
+        size_t tsz = (size_t)std::max(A.nnz, 1);
+        DBuf<int> tcol(c, tsz), uniq(c, (size_t)n + 1);
+        DBuf<double> tval(c, tsz);
+        unit_product_in_place(c, A, B, tcol.get(), tval.get(), uniq.get(), nullptr);
+        spgemm_finish<int>(c, n, uniq.get(), A.row.get(), tcol.get(), tval.get(), C);
+        return;
+    }
+    bool unitb = B.unit_rows && B.nnz == B.n;  // a prolongator: one product per entry of A
+    spgemm_rows(c, A, BRows{B.n, B.m, B.row.get(), B.row.get() + 1, B.col.get(), B.val.get(), unitb}, C);
+}
+
+// The Galerkin product R (A P).  When A P fits the one-launch product its rows
+// stay where that launch leaves them (in A's entry positions, gaps after each)
+// and R reads them there: A P is never compacted or measured on the host.
+void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C) {
+    BCMGX_CHECK(A.m == P.n && R.m == A.n, BCMGX_ERR_ARG, "galerkin: dimensions differ");
+    if (A.n > 0 && R.n > 0 && unit_product_fits(A, P)) {
+        size_t tsz = (size_t)std::max(A.nnz, 1);
+        DBuf<int> tcol(c, tsz), rend(c, (size_t)A.n);
+        DBuf<double> tval(c, tsz);
+        unit_product_in_place(c, A, P, tcol.get(), tval.get(), nullptr, rend.get());
+        C.n = R.n;
+        C.m = P.m;
+        C.unit_rows = false;
+        spgemm_rows(c, R, BRows{A.n, P.m, A.row.get(), rend.get(), tcol.get(), tval.get(), false}, C);
+        return;
+    }
+    DCsr AP;
+    spgemm(c, A, P, AP);
+    spgemm(c, R, AP, C);
 }
 
 // ------------------------------------------------------- row-tile index ---
