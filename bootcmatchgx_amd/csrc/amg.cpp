@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <type_traits>
 
 namespace bcmgx {
@@ -166,8 +167,11 @@ static DBuf<int> check_diagonals(Context* c, const DCsr& A) {
 }
 
 // w0: the smooth vector the matching weights start from (nullptr: w = 1, as
-// BootBuildData::initByParams fills it, AMG.cu:383-384)
-static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H, const double* w0 = nullptr) {
+// BootBuildData::initByParams fills it, AMG.cu:383-384).  level_done(l) is
+// called once level l is final (its matrix, M, aggregates and R), while the
+// levels below it are still being built.
+static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H, const double* w0 = nullptr,
+                            const std::function<void(int)>& level_done = {}) {
     BCMGX_CHECK(A0.n == A0.m, BCMGX_ERR_ARG, "the matrix must be square");
     DBuf<int> dind0 = check_diagonals(c, A0);
     int maxcs = p.max_coarse_size > 0 ? p.max_coarse_size : reference_max_coarse_size(A0.n);
@@ -235,9 +239,11 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
         avc += (double)prev.mat().n / (double)nc;
         H.L.push_back(std::move(nl));
         level++;
+        if (level_done) level_done(level - 2);
         if ((double)nc <= (double)ftcoarse * maxcs) break;
     }
     int nlv = H.levels();
+    if (level_done) level_done(nlv - 1);
     double nnz0 = (double)H.L[0]->mat().nnz, cx = 0, cw = 0;
     for (int i = 0; i < nlv; i++) {
         cx += H.L[i]->mat().nnz;
@@ -261,29 +267,83 @@ static void make_level_product(Context* c, Hierarchy& H, int l) {
     lv.AP.val.release();
     lv.lanesAP = default_lanes(lv.AP.nnz, lv.AP.n);
 }
-static void make_level_products(Context* c, Hierarchy& H) {
-    for (int l = 0; l + 1 < H.levels(); l++) make_level_product(c, H, l);
+// The same product queued with no host synchronisation (false when A_l's rows
+// are too long for it); level_product_length reads its length afterwards
+static bool level_product_async(Context* c, Hierarchy& H, int l) {
+    Level& lv = *H.L[l];
+    return unit_product_f32_async(c, lv.mat(), H.L[l + 1]->mat().n, lv.agg.get(), lv.pval.get(), lv.AP);
+}
+static void level_product_length(Context* c, Level& lv) {
+    lv.AP.nnz = c->read1(lv.AP.row.get() + lv.AP.n);
+    lv.lanesAP = default_lanes(lv.AP.nnz, lv.AP.n);
 }
 
-static void make_fp32_copies(Context* c, Hierarchy& H) {
-    for (auto& lp : H.L) {
-        Level& l = *lp;
-        const DCsr& A = l.mat();
-        // the FP32 values live beside the FP64 ones; level 0's (the caller's
-        // matrix) are converted again by every setup, as the coarse levels' are
-        DCsr& Am = const_cast<DCsr&>(A);
-        if (!Am.valf) Am.valf.reset(c, std::max(A.nnz, 1));
-        convert<double, float>(c, A.val.get(), Am.valf.get(), A.nnz);
-        l.Mf.reset(c, std::max(A.n, 1));
-        convert<double, float>(c, l.M.get(), l.Mf.get(), A.n);
-        if (l.pval) {
-            l.pvalf.reset(c, l.pval.size());
-            convert<double, float>(c, l.pval.get(), l.pvalf.get(), (long long)l.pval.size());
-            l.R.valf.reset(c, std::max(l.R.nnz, 1));
-            convert<double, float>(c, l.R.val.get(), l.R.valf.get(), l.R.nnz);
-        }
+static void level_fp32_copy(Context* c, Level& l) {
+    const DCsr& A = l.mat();
+    // the FP32 values live beside the FP64 ones; level 0's (the caller's
+    // matrix) are converted again by every setup, as the coarse levels' are
+    DCsr& Am = const_cast<DCsr&>(A);
+    if (!Am.valf) Am.valf.reset(c, std::max(A.nnz, 1));
+    convert<double, float>(c, A.val.get(), Am.valf.get(), A.nnz);
+    l.Mf.reset(c, std::max(A.n, 1));
+    convert<double, float>(c, l.M.get(), l.Mf.get(), A.n);
+    if (l.pval) {
+        l.pvalf.reset(c, l.pval.size());
+        convert<double, float>(c, l.pval.get(), l.pvalf.get(), (long long)l.pval.size());
+        l.R.valf.reset(c, std::max(l.R.nnz, 1));
+        convert<double, float>(c, l.R.val.get(), l.R.valf.get(), l.R.nnz);
     }
 }
+
+// Setup work queued on the context's second stream, behind everything queued
+// on the first so far, so that it overlaps the latency-bound matching of the
+// next levels.  Blocks the queued work frees are held back by the pool until
+// join() has made the first stream wait for it; the destructor waits for the
+// second stream when a setup ends early (an error between run() and join()).
+struct SideQueue {
+    Context* c;
+    bool pending = false;
+    explicit SideQueue(Context* c_) : c(c_) {}
+    SideQueue(const SideQueue&) = delete;
+    SideQueue& operator=(const SideQueue&) = delete;
+    ~SideQueue() {
+        if (pending) {
+            (void)hipStreamSynchronize(c->side);
+            c->pool->release_deferred();
+        }
+    }
+    static void order(hipStream_t before, hipStream_t after) {
+        hipEvent_t e;
+        BCMGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BCMGX_HIP(hipEventRecord(e, before));
+        hipError_t r = hipStreamWaitEvent(after, e, 0);
+        (void)hipEventDestroy(e);
+        BCMGX_HIP(r);
+    }
+    template <typename F>
+    void run(F&& f) {
+        hipStream_t side = c->side_stream(), main = c->stream;
+        order(main, side);
+        pending = true;
+        c->stream = side;
+        c->pool->set_defer(true);
+        try {
+            f();
+        } catch (...) {
+            c->stream = main;
+            c->pool->set_defer(false);
+            throw;
+        }
+        c->stream = main;
+        c->pool->set_defer(false);
+    }
+    void join() {
+        if (!pending) return;
+        order(c->side, c->stream);
+        c->pool->release_deferred();
+        pending = false;
+    }
+};
 
 // ------------------------------------------------------------ prec ----
 struct SolveWork {
@@ -767,9 +827,29 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
         BCMGX_CHECK(p.max_hrc >= 1, BCMGX_ERR_PARAM, "max_hrc must be at least 1");
         BCMGX_CHECK(p.cycle_type >= 0 && p.cycle_type < BCMGX_CYCLE_INVALID, BCMGX_ERR_PARAM, "invalid cycle_type");
         bool f32 = p.precision == BCMGX_PRECISION_FP32;
+        // a level's FP32 copies and A_l P_l go to the second stream as soon as
+        // the level is final; finish() joins them
+        SideQueue side(c);
+        std::vector<char> async_product;
+        auto level_done = [&](Hierarchy& H) {
+            async_product.assign(64, 0);
+            Hierarchy* hp = &H;
+            return [&, hp](int l) {
+                if (!f32) return;
+                side.run([&] {
+                    level_fp32_copy(c, *hp->L[l]);
+                    if (p.post_galerkin && l + 1 < hp->levels() && l < (int)async_product.size())
+                        async_product[l] = level_product_async(c, *hp, l) ? 1 : 0;
+                });
+            };
+        };
         auto finish = [&](Hierarchy& H) {
-            if (f32) make_fp32_copies(c, H);
-            if (f32 && p.post_galerkin) make_level_products(c, H);
+            side.join();
+            if (f32 && p.post_galerkin)
+                for (int l = 0; l + 1 < H.levels(); l++) {
+                    if (l < (int)async_product.size() && async_product[l]) level_product_length(c, *H.L[l]);
+                    else make_level_product(c, H, l);
+                }
             for (auto& lp : H.L) {
                 Level& l = *lp;
                 const DCsr& M = l.mat();
@@ -777,12 +857,13 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
                 l.lanesR = l.R.n > 0 ? default_lanes(l.R.nnz, l.R.n) : 1;
             }
         };
-        build_hierarchy(c, A, p, pr->H);
+        build_hierarchy(c, A, p, pr->H, nullptr, level_done(pr->H));
         finish(pr->H);
         bootstrap(c, A, pr.get(), [&](const double* w) {
             pr->HB.emplace_back(new Hierarchy());
-            build_hierarchy(c, A, p, *pr->HB.back(), w);
-            finish(*pr->HB.back());
+            Hierarchy& H = *pr->HB.back();
+            build_hierarchy(c, A, p, H, w, level_done(H));
+            finish(H);
         });
         break;
     }

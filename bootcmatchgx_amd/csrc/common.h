@@ -58,10 +58,17 @@ public:
     void free(void* p);
     void release_all();
     size_t bytes_in_use() const { return in_use_; }
+    // Blocks are reused in the order of one stream.  While work is queued on a
+    // second stream, the blocks that work frees are held back (set_defer(true))
+    // until the first stream has been made to wait for it (release_deferred()).
+    void set_defer(bool d) { defer_ = d; }
+    void release_deferred();
 
 private:
     static size_t size_class(size_t b);
     int device_;
+    bool defer_ = false;
+    std::vector<std::pair<size_t, void*>> deferred_;
     std::multimap<size_t, void*> free_;
     std::map<void*, size_t> live_;
     size_t in_use_ = 0;
@@ -103,6 +110,7 @@ private:
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // a second stream (default priority) for setup work that overlaps, made on first use
     std::unique_ptr<DevicePool> pool;
     int num_cus = 256;
     int num_xcds = 8;
@@ -118,6 +126,7 @@ struct Context {
     explicit Context(int dev);
     ~Context();
     void sync() { BCMGX_HIP(hipStreamSynchronize(stream)); }
+    hipStream_t side_stream();
     template <typename T>
     T* dalloc(size_t n) { return static_cast<T*>(pool->alloc(n * sizeof(T))); }
     void dfree(void* p) { pool->free(p); }

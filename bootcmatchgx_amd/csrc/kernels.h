@@ -238,6 +238,9 @@ void transpose(Context* c, const DCsr& A, DCsr& T);
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
 // C = R (A P), the same bits as spgemm(R, spgemm(A, P)); A P is not compacted when P is a prolongator
 void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C);
+// A P with FP32 values for the prolongator (agg, pval) onto nc columns, queued with no host
+// synchronisation (C.nnz = -1: the length is at C.row[n]); false when A's rows are too long
+bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, DCsr& C);
 // P <- P P_i for one-entry-per-row operators: agg[k] = agg_i[agg[k]], pval[k] *= pval_i[agg[k]]
 void compose_prolongators(Context* c, int n, int* agg, double* pval, const int* agg_i, const double* pval_i);
 // csr of a one-entry-per-row prolongator (row = identity)

@@ -55,11 +55,18 @@ void DevicePool::free(void* p) {
     auto it = live_.find(p);
     if (it == live_.end()) return;
     in_use_ -= it->second;
-    free_.insert({it->second, p});
+    if (defer_) deferred_.push_back({it->second, p});
+    else free_.insert({it->second, p});
     live_.erase(it);
 }
 
+void DevicePool::release_deferred() {
+    for (auto& d : deferred_) free_.insert(d);
+    deferred_.clear();
+}
+
 void DevicePool::release_all() {
+    release_deferred();
     if (!free_.empty() || !live_.empty()) (void)hipSetDevice(device_);
     for (auto& f : free_) (void)hipFree(f.second);
     free_.clear();
@@ -80,10 +87,17 @@ Context::Context(int dev) : device(dev) {
 }
 
 Context::~Context() {
+    if (side) (void)hipStreamSynchronize(side);
     if (stream) (void)hipStreamSynchronize(stream);
     pool.reset();
     if (pinned) (void)hipHostFree(pinned);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+hipStream_t Context::side_stream() {
+    if (!side) BCMGX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    return side;
 }
 
 void* Context::host_scratch(size_t bytes) {

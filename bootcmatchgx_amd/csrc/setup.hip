@@ -1133,10 +1133,10 @@ __global__ __launch_bounds__(WG) void k_spgemm_large(int nrows, const int* __res
                   gpre + aoff[r], lscan);
 }
 
-template <typename O>
+template <typename O, typename V>
 __global__ void k_spgemm_compact(int n, const O* __restrict__ poff, const int* __restrict__ crow,
                                  const int* __restrict__ tcol, const double* __restrict__ tval, int* __restrict__ ccol,
-                                 double* __restrict__ cval) {
+                                 V* __restrict__ cval) {
     int i = (blockIdx.x * WG + (int)threadIdx.x) / 16;
     int lane = (int)threadIdx.x % 16;
     if (i >= n) return;
@@ -1144,7 +1144,7 @@ __global__ void k_spgemm_compact(int n, const O* __restrict__ poff, const int* _
     int dst = crow[i], len = crow[i + 1] - crow[i];
     for (int t = lane; t < len; t += 16) {
         ccol[dst + t] = tcol[src + t];
-        cval[dst + t] = tval[src + t];
+        cval[dst + t] = (V)tval[src + t];
     }
 }
 
@@ -1177,7 +1177,8 @@ static void spgemm_finish(Context* c, int n, const int* uniq, const O* poff, con
     C.maxrow = maxrow;
     C.col.reset(c, (size_t)std::max(nnz, 1));
     C.val.reset(c, (size_t)std::max(nnz, 1));
-    k_spgemm_compact<O><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff, C.row.get(), tcol, tval, C.col.get(), C.val.get());
+    k_spgemm_compact<O, double><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff, C.row.get(), tcol, tval, C.col.get(),
+                                                                           C.val.get());
     BCMGX_LAUNCHED();
 }
 
@@ -1291,6 +1292,37 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
     }
     bool unitb = B.unit_rows && B.nnz == B.n;  // a prolongator: one product per entry of A
     spgemm_rows(c, A, BRows{B.n, B.m, B.row.get(), B.row.get() + 1, B.col.get(), B.val.get(), unitb}, C);
+}
+
+// A P for the prolongator (agg, pval) of A's n rows onto nc, when every row of A
+// fits the one-launch product: row pointers, columns and values rounded to FP32
+// (C.valf; no FP64 values) queued on c's stream with no host synchronisation.
+// The result's length stays on the device at C.row[n] (C.nnz = -1 until read);
+// the columns and values are sized for A's entries.  False (nothing queued)
+// when A does not fit.
+bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, DCsr& C) {
+    int n = A.n;
+    if (n <= 0 || nc >= (1 << 26) || A.maxrow < 0 || A.maxrow > 32) return false;
+    DCsr P;
+    prolongator_csr(c, n, nc, agg, pval, P);
+    size_t tsz = (size_t)std::max(A.nnz, 1);
+    DBuf<int> tcol(c, tsz), uniq(c, (size_t)n + 1);
+    DBuf<double> tval(c, tsz);
+    unit_product_in_place(c, A, P, tcol.get(), tval.get(), uniq.get(), nullptr);
+    C.n = n;
+    C.m = nc;
+    C.nnz = -1;
+    C.unit_rows = false;
+    C.maxrow = -1;
+    C.row.reset(c, (size_t)n + 1);
+    scan_exclusive_async(c, uniq.get(), C.row.get(), n);
+    C.col.reset(c, tsz);
+    C.val.release();
+    C.valf.reset(c, tsz);
+    k_spgemm_compact<int, float><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, A.row.get(), C.row.get(), tcol.get(),
+                                                                           tval.get(), C.col.get(), C.valf.get());
+    BCMGX_LAUNCHED();
+    return true;
 }
 
 // The Galerkin product R (A P).  When A P fits the one-launch product its rows
