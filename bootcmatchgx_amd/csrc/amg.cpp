@@ -206,7 +206,7 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             DCsr Pi, Ri, Anew;
             prolongator_csr(c, n, nc, agg_i.get(), pval_i.get(), Pi);
             transpose_prolongator(c, n, nc, agg_i.get(), pval_i.get(), Ri);
-            galerkin(c, *Ai, Pi, Ri, Anew);
+            galerkin(c, *Ai, Pi, Ri, Anew, 2);  // pairs and singletons: at most 2 entries a row of R
             DBuf<double> wn(c, nc);
             {
                 SpmvArgs a;  // w_{i+1} = R_i w_i, one lane per row (sequential sums)

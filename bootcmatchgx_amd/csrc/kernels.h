@@ -236,8 +236,9 @@ void transpose_prolongator(Context* c, int n, int nc, const int* agg, const doub
 void transpose(Context* c, const DCsr& A, DCsr& T);
 // C = A B, deterministic ascending-k accumulation, sorted columns
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
-// C = R (A P), the same bits as spgemm(R, spgemm(A, P)); A P is not compacted when P is a prolongator
-void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C);
+// C = R (A P), the same bits as spgemm(R, spgemm(A, P)); A P is not compacted when P is a
+// prolongator.  rmax: a bound on the entries of R's rows (-1: none known)
+void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C, int rmax = -1);
 // A P with FP32 values for the prolongator (agg, pval) onto nc columns, queued with no host
 // synchronisation (C.nnz = -1: the length is at C.row[n]); false when A's rows are too long
 bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, DCsr& C);

@@ -1325,10 +1325,19 @@ bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, c
     return true;
 }
 
+__global__ void k_stride_offsets(int n, int stride, int* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i * stride;
+}
+
 // The Galerkin product R (A P).  When A P fits the one-launch product its rows
 // stay where that launch leaves them (in A's entry positions, gaps after each)
-// and R reads them there: A P is never compacted or measured on the host.
-void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C) {
+// and R reads them there: A P is never compacted or measured on the host.  Its
+// rows then hold at most A.maxrow entries, so when R's rows hold at most rmax
+// (the caller's bound: 2 for a pairwise aggregation's restriction) and
+// rmax A.maxrow <= 64, every row of the product fits one 64-lane group: one
+// launch over all rows, 64 product slots per row, no counting or binning.
+void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C, int rmax) {
     BCMGX_CHECK(A.m == P.n && R.m == A.n, BCMGX_ERR_ARG, "galerkin: dimensions differ");
     if (A.n > 0 && R.n > 0 && unit_product_fits(A, P)) {
         size_t tsz = (size_t)std::max(A.nnz, 1);
@@ -1338,6 +1347,19 @@ void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C) 
         C.n = R.n;
         C.m = P.m;
         C.unit_rows = false;
+        if (rmax >= 0 && rmax * A.maxrow <= SMALL_P && (long long)R.n * SMALL_P < (1ll << 31)) {
+            size_t slots = (size_t)R.n * SMALL_P;
+            DBuf<int> off(c, (size_t)R.n), ccol(c, slots), uniq(c, (size_t)R.n + 1);
+            DBuf<double> cval(c, slots);
+            k_stride_offsets<<<grid_for(R.n, WG), WG, 0, c->stream>>>(R.n, SMALL_P, off.get());
+            BCMGX_LAUNCHED();
+            k_spgemm_small<SMALL_P, true, false><<<grid_for(R.n, WG / SMALL_P), WG, 0, c->stream>>>(
+                R.n, nullptr, R.row.get(), R.col.get(), R.val.get(), A.row.get(), rend.get(), tcol.get(), tval.get(), nullptr,
+                ccol.get(), cval.get(), uniq.get(), off.get(), nullptr);
+            BCMGX_LAUNCHED();
+            spgemm_finish<int>(c, R.n, uniq.get(), off.get(), ccol.get(), cval.get(), C);
+            return;
+        }
         spgemm_rows(c, R, BRows{A.n, P.m, A.row.get(), rend.get(), tcol.get(), tval.get(), false}, C);
         return;
     }
