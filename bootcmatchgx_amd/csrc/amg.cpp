@@ -200,7 +200,15 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             int n = Ai->n;
             DBuf<int> M(c, n), agg_i(c, n);
             DBuf<double> pval_i(c, n);
-            suitor_match(c, *Ai, wi.get(), M.get(), Ai == &A0 ? dind0.get() : nullptr);
+            // A Galerkin product keeps structural entries, so every row of R (A P) holds a
+            // diagonal when every row of A does (aggregate c's row gets a_ii of its members):
+            // only the fine matrix needs the check
+            DBuf<int> dind_i;
+            if (Ai != &A0) {
+                dind_i.reset(c, std::max(n, 1));
+                diag_index(c, n, Ai->row.get(), Ai->col.get(), dind_i.get());
+            }
+            suitor_match(c, *Ai, wi.get(), M.get(), Ai == &A0 ? dind0.get() : dind_i.get());
             int nc = make_prolongator(c, n, M.get(), wi.get(), agg_i.get(), pval_i.get());
             BCMGX_CHECK(nc > 0, BCMGX_ERR_NUMERIC, "coarsening produced an empty level");
             DCsr Pi, Ri, Anew;
