@@ -154,6 +154,56 @@ int tune_chunk(Context* c, const DCsr& A, bool fp32, int k) {
 // setMaxCoarseSize (AMG.cu:238-241), truncation included
 static int reference_max_coarse_size(long long n) { return (int)(40 * std::pow((double)n, (double)1. / 3.)); }
 
+// Setup work queued on the context's second stream, behind everything queued
+// on the first so far, so that it overlaps the latency-bound matching of the
+// next levels.  Blocks the queued work frees are held back by the pool until
+// join() has made the first stream wait for it; the destructor waits for the
+// second stream when a setup ends early (an error between run() and join()).
+struct SideQueue {
+    Context* c;
+    bool pending = false;
+    explicit SideQueue(Context* c_) : c(c_) {}
+    SideQueue(const SideQueue&) = delete;
+    SideQueue& operator=(const SideQueue&) = delete;
+    ~SideQueue() {
+        if (pending) {
+            (void)hipStreamSynchronize(c->side);
+            c->pool->release_deferred();
+        }
+    }
+    static void order(hipStream_t before, hipStream_t after) {
+        hipEvent_t e;
+        BCMGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BCMGX_HIP(hipEventRecord(e, before));
+        hipError_t r = hipStreamWaitEvent(after, e, 0);
+        (void)hipEventDestroy(e);
+        BCMGX_HIP(r);
+    }
+    template <typename F>
+    void run(F&& f) {
+        hipStream_t side = c->side_stream(), main = c->stream;
+        order(main, side);
+        pending = true;
+        c->stream = side;
+        c->pool->set_defer(true);
+        try {
+            f();
+        } catch (...) {
+            c->stream = main;
+            c->pool->set_defer(false);
+            throw;
+        }
+        c->stream = main;
+        c->pool->set_defer(false);
+    }
+    void join() {
+        if (!pending) return;
+        order(c->side, c->stream);
+        c->pool->release_deferred();
+        pending = false;
+    }
+};
+
 // A's diagonal positions, every row checked to have one
 static DBuf<int> check_diagonals(Context* c, const DCsr& A) {
     DBuf<int> dind(c, std::max(A.n, 1));
@@ -167,11 +217,12 @@ static DBuf<int> check_diagonals(Context* c, const DCsr& A) {
 }
 
 // w0: the smooth vector the matching weights start from (nullptr: w = 1, as
-// BootBuildData::initByParams fills it, AMG.cu:383-384).  level_done(l) is
-// called once level l is final (its matrix, M, aggregates and R), while the
-// levels below it are still being built.
+// BootBuildData::initByParams fills it, AMG.cu:383-384).  Once level l is final
+// (its matrix, aggregates and R) its l1 row sums M are computed and then
+// level_done(l) is called, both on the side queue when one is given (the
+// levels below are still being built meanwhile) or else in order.
 static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H, const double* w0 = nullptr,
-                            const std::function<void(int)>& level_done = {}) {
+                            SideQueue* side = nullptr, const std::function<void(int)>& level_done = {}) {
     BCMGX_CHECK(A0.n == A0.m, BCMGX_ERR_ARG, "the matrix must be square");
     DBuf<int> dind0 = check_diagonals(c, A0);
     int maxcs = p.max_coarse_size > 0 ? p.max_coarse_size : reference_max_coarse_size(A0.n);
@@ -180,8 +231,17 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
     H.L.clear();
     H.L.emplace_back(new Level());
     H.L[0]->Aref = &A0;
-    H.L[0]->M.reset(c, A0.n);
-    abs_row_sum(c, A0.n, A0.row.get(), A0.val.get(), H.L[0]->M.get());
+    auto final_level = [&](int l) {
+        auto work = [&] {
+            Level& lv = *H.L[l];
+            const DCsr& M = lv.mat();
+            lv.M.reset(c, std::max(M.n, 1));
+            abs_row_sum(c, M.n, M.row.get(), M.val.get(), lv.M.get());
+            if (level_done) level_done(l);
+        };
+        if (side) side->run(work);
+        else work();
+    };
 
     DBuf<double> w(c, A0.n);
     if (w0) copy_d2d(c, w.get(), w0, sizeof(double) * (size_t)A0.n);
@@ -238,8 +298,6 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
         w = std::move(wi);
         std::unique_ptr<Level> nl(new Level());
         nl->A = std::move(Aown);
-        nl->M.reset(c, nl->A.n);
-        abs_row_sum(c, nl->A.n, nl->A.row.get(), nl->A.val.get(), nl->M.get());
         int nc = nl->A.n;
         prev.agg = std::move(agg);
         prev.pval = std::move(pv);
@@ -247,11 +305,11 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
         avc += (double)prev.mat().n / (double)nc;
         H.L.push_back(std::move(nl));
         level++;
-        if (level_done) level_done(level - 2);
+        final_level(level - 2);
         if ((double)nc <= (double)ftcoarse * maxcs) break;
     }
     int nlv = H.levels();
-    if (level_done) level_done(nlv - 1);
+    final_level(nlv - 1);
     double nnz0 = (double)H.L[0]->mat().nnz, cx = 0, cw = 0;
     for (int i = 0; i < nlv; i++) {
         cx += H.L[i]->mat().nnz;
@@ -276,14 +334,22 @@ static void make_level_product(Context* c, Hierarchy& H, int l) {
     lv.lanesAP = default_lanes(lv.AP.nnz, lv.AP.n);
 }
 // The same product queued with no host synchronisation (false when A_l's rows
-// are too long for it); level_product_length reads its length afterwards
+// are too long for it); level_product_lengths reads the lengths afterwards
 static bool level_product_async(Context* c, Hierarchy& H, int l) {
     Level& lv = *H.L[l];
     return unit_product_f32_async(c, lv.mat(), H.L[l + 1]->mat().n, lv.agg.get(), lv.pval.get(), lv.AP);
 }
-static void level_product_length(Context* c, Level& lv) {
-    lv.AP.nnz = c->read1(lv.AP.row.get() + lv.AP.n);
-    lv.lanesAP = default_lanes(lv.AP.nnz, lv.AP.n);
+static void level_product_lengths(Context* c, const std::vector<Level*>& lvs) {
+    if (lvs.empty()) return;
+    int* h = (int*)c->host_scratch(lvs.size() * sizeof(int));
+    for (size_t t = 0; t < lvs.size(); t++)
+        BCMGX_HIP(hipMemcpyAsync(h + t, lvs[t]->AP.row.get() + lvs[t]->AP.n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    c->sync();  // one round trip for every level
+    for (size_t t = 0; t < lvs.size(); t++) {
+        Level& lv = *lvs[t];
+        lv.AP.nnz = h[t];
+        lv.lanesAP = default_lanes(lv.AP.nnz, lv.AP.n);
+    }
 }
 
 static void level_fp32_copy(Context* c, Level& l) {
@@ -303,55 +369,6 @@ static void level_fp32_copy(Context* c, Level& l) {
     }
 }
 
-// Setup work queued on the context's second stream, behind everything queued
-// on the first so far, so that it overlaps the latency-bound matching of the
-// next levels.  Blocks the queued work frees are held back by the pool until
-// join() has made the first stream wait for it; the destructor waits for the
-// second stream when a setup ends early (an error between run() and join()).
-struct SideQueue {
-    Context* c;
-    bool pending = false;
-    explicit SideQueue(Context* c_) : c(c_) {}
-    SideQueue(const SideQueue&) = delete;
-    SideQueue& operator=(const SideQueue&) = delete;
-    ~SideQueue() {
-        if (pending) {
-            (void)hipStreamSynchronize(c->side);
-            c->pool->release_deferred();
-        }
-    }
-    static void order(hipStream_t before, hipStream_t after) {
-        hipEvent_t e;
-        BCMGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        BCMGX_HIP(hipEventRecord(e, before));
-        hipError_t r = hipStreamWaitEvent(after, e, 0);
-        (void)hipEventDestroy(e);
-        BCMGX_HIP(r);
-    }
-    template <typename F>
-    void run(F&& f) {
-        hipStream_t side = c->side_stream(), main = c->stream;
-        order(main, side);
-        pending = true;
-        c->stream = side;
-        c->pool->set_defer(true);
-        try {
-            f();
-        } catch (...) {
-            c->stream = main;
-            c->pool->set_defer(false);
-            throw;
-        }
-        c->stream = main;
-        c->pool->set_defer(false);
-    }
-    void join() {
-        if (!pending) return;
-        order(c->side, c->stream);
-        c->pool->release_deferred();
-        pending = false;
-    }
-};
 
 // ------------------------------------------------------------ prec ----
 struct SolveWork {
@@ -842,22 +859,23 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
         auto level_done = [&](Hierarchy& H) {
             async_product.assign(64, 0);
             Hierarchy* hp = &H;
-            return [&, hp](int l) {
+            return [&, hp](int l) {  // runs on the side queue
                 if (!f32) return;
-                side.run([&] {
-                    level_fp32_copy(c, *hp->L[l]);
-                    if (p.post_galerkin && l + 1 < hp->levels() && l < (int)async_product.size())
-                        async_product[l] = level_product_async(c, *hp, l) ? 1 : 0;
-                });
+                level_fp32_copy(c, *hp->L[l]);
+                if (p.post_galerkin && l + 1 < hp->levels() && l < (int)async_product.size())
+                    async_product[l] = level_product_async(c, *hp, l) ? 1 : 0;
             };
         };
         auto finish = [&](Hierarchy& H) {
             side.join();
-            if (f32 && p.post_galerkin)
+            if (f32 && p.post_galerkin) {
+                std::vector<Level*> queued;
                 for (int l = 0; l + 1 < H.levels(); l++) {
-                    if (l < (int)async_product.size() && async_product[l]) level_product_length(c, *H.L[l]);
+                    if (l < (int)async_product.size() && async_product[l]) queued.push_back(H.L[l].get());
                     else make_level_product(c, H, l);
                 }
+                level_product_lengths(c, queued);
+            }
             for (auto& lp : H.L) {
                 Level& l = *lp;
                 const DCsr& M = l.mat();
@@ -865,12 +883,12 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
                 l.lanesR = l.R.n > 0 ? default_lanes(l.R.nnz, l.R.n) : 1;
             }
         };
-        build_hierarchy(c, A, p, pr->H, nullptr, level_done(pr->H));
+        build_hierarchy(c, A, p, pr->H, nullptr, &side, level_done(pr->H));
         finish(pr->H);
         bootstrap(c, A, pr.get(), [&](const double* w) {
             pr->HB.emplace_back(new Hierarchy());
             Hierarchy& H = *pr->HB.back();
-            build_hierarchy(c, A, p, H, w, level_done(H));
+            build_hierarchy(c, A, p, H, w, &side, level_done(H));
             finish(H);
         });
         break;
