@@ -218,7 +218,7 @@ static DBuf<int> check_diagonals(Context* c, const DCsr& A) {
 
 // w0: the smooth vector the matching weights start from (nullptr: w = 1, as
 // BootBuildData::initByParams fills it, AMG.cu:383-384).  Once level l is final
-// (its matrix, aggregates and R) its l1 row sums M are computed and then
+// (its matrix and aggregates) its l1 row sums M and restriction R are computed and then
 // level_done(l) is called, both on the side queue when one is given (the
 // levels below are still being built meanwhile) or else in order.
 static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, Hierarchy& H, const double* w0 = nullptr,
@@ -237,6 +237,8 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             const DCsr& M = lv.mat();
             lv.M.reset(c, std::max(M.n, 1));
             abs_row_sum(c, M.n, M.row.get(), M.val.get(), lv.M.get());
+            if (l + 1 < H.levels())  // the restriction of the composed prolongator
+                transpose_prolongator(c, M.n, H.L[l + 1]->mat().n, lv.agg.get(), lv.pval.get(), lv.R);
             if (level_done) level_done(l);
         };
         if (side) side->run(work);
@@ -301,7 +303,6 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
         int nc = nl->A.n;
         prev.agg = std::move(agg);
         prev.pval = std::move(pv);
-        transpose_prolongator(c, prev.mat().n, nc, prev.agg.get(), prev.pval.get(), prev.R);
         avc += (double)prev.mat().n / (double)nc;
         H.L.push_back(std::move(nl));
         level++;
