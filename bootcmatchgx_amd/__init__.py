@@ -276,6 +276,15 @@ class CSR:
         return o
 
     @classmethod
+    def from_device(cls, ctx, n, m, nnz, row_ptr, col_ptr, val_ptr):
+        """A matrix copied from CSR arrays already in device memory (int32 row pointers
+        and columns, float64 values; addresses as ints or c_void_p)."""
+        h = C.c_void_p()
+        vp = lambda x: x if isinstance(x, C.c_void_p) else C.c_void_p(x)
+        _check(lib().bcmgx_csr_create(ctx.h, n, m, nnz, vp(row_ptr), vp(col_ptr), vp(val_ptr), MEM_DEVICE, C.byref(h)), ctx)
+        return cls._wrap(ctx, h)
+
+    @classmethod
     def from_scipy(cls, ctx, S):
         S = S.tocsr()
         S.sort_indices()

@@ -208,7 +208,8 @@ int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, c
             BCMGX_HIP(hipMemcpyAsync(h->A.col.get(), col, sizeof(int) * (size_t)nnz, k, c->stream));
             BCMGX_HIP(hipMemcpyAsync(h->A.val.get(), val, sizeof(double) * (size_t)nnz, k, c->stream));
         }
-        if (mem == BCMGX_MEM_HOST) {  // shape facts the kernels use: the longest row, one entry per row
+        // shape facts the kernels use: the longest row, one entry per row
+        if (mem == BCMGX_MEM_HOST) {
             int mx = 0;
             for (int i = 0; i < n; i++) mx = std::max(mx, row[i + 1] - row[i]);
             h->A.maxrow = mx;
@@ -217,8 +218,10 @@ int bcmgx_csr_create(bcmgx_context ctx, int n, int m, int nnz, const int* row, c
                 for (int i = 0; i <= n && unit; i++) unit = row[i] == i;
                 h->A.unit_rows = unit;
             }
+            c->sync();
+        } else {
+            csr_shape(c, h->A);  // reads back after the copies (same stream)
         }
-        c->sync();
     } catch (...) {
         delete h;
         throw;

@@ -236,6 +236,8 @@ void transpose_prolongator(Context* c, int n, int nc, const int* agg, const doub
 void transpose(Context* c, const DCsr& A, DCsr& T);
 // C = A B, deterministic ascending-k accumulation, sorted columns
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
+// A.maxrow and A.unit_rows from A's row pointers on the device (one read-back)
+void csr_shape(Context* c, DCsr& A);
 // C = R (A P), the same bits as spgemm(R, spgemm(A, P)); A P is not compacted when P is a
 // prolongator.  rmax: a bound on the entries of R's rows (-1: none known)
 void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C, int rmax = -1);

@@ -739,6 +739,51 @@ __global__ __launch_bounds__(WG) void k_max_int(int n, const int* __restrict__ v
     }
 }
 
+// the longest row into out[0] and the number of rows i whose pointer is not i into
+// out[1] (both hold 0 on entry): a grid-stride pass, one atomic of each per block
+__global__ __launch_bounds__(WG) void k_row_shape(int n, const int* __restrict__ row, int* __restrict__ out) {
+    __shared__ int part[2][WG / 64];
+    int mx = 0, bad = 0;
+    for (int i = blockIdx.x * WG + (int)threadIdx.x; i < n; i += gridDim.x * WG) {
+        int a = row[i], b = row[i + 1];
+        mx = b - a > mx ? b - a : mx;
+        bad += a != i ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int y = __shfl_xor(mx, o, 64);
+        mx = y > mx ? y : mx;
+        bad += __shfl_xor(bad, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = mx;
+        part[1][threadIdx.x >> 6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < WG / 64; w++) {
+            mx = part[0][w] > mx ? part[0][w] : mx;
+            bad += part[1][w];
+        }
+        if (mx > 0) atomicMax(out, mx);
+        if (bad > 0) atomicAdd(out + 1, bad);
+    }
+}
+
+void csr_shape(Context* c, DCsr& A) {
+    A.maxrow = 0;
+    A.unit_rows = false;
+    if (A.n <= 0) return;
+    DBuf<int> out(c, 2);
+    fill<int>(c, out.get(), 2, 0);
+    k_row_shape<<<std::min<unsigned>(grid_for(A.n, WG), MAX_INT_BLOCKS), WG, 0, c->stream>>>(A.n, A.row.get(), out.get());
+    BCMGX_LAUNCHED();
+    int h[2];
+    c->d2h(h, out.get(), 2);
+    A.maxrow = h[0];
+    A.unit_rows = A.nnz == A.n && h[1] == 0;
+}
+
 __global__ void k_row_lengths(int n, const int* __restrict__ arow, long long* __restrict__ cnt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) cnt[i] = arow[i + 1] - arow[i];

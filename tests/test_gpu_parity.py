@@ -227,6 +227,57 @@ def test_hierarchy_bit_exact_harwell_boeing(ctx, golden, name):
             assert np.array_equal(a1, a2) and np.array_equal(p1, p2), l
 
 
+def _dev_from_device(ctx, ocsr):
+    """The oracle matrix copied into device buffers, then created from device memory."""
+    n, m, _ = ocsr.dims()
+    r, c, v = ocsr.arrays()
+    bufs = []
+
+    def put(a):
+        raw = np.ascontiguousarray(a).tobytes()
+        raw += b"\0" * (-len(raw) % 8)
+        d = bx.DeviceArray(ctx, (max(len(raw) // 8, 1),))
+        if raw:
+            d.upload(np.frombuffer(raw, np.float64))
+        bufs.append(d)
+        return d.ptr
+
+    A = bx.CSR.from_device(ctx, n, m, len(c), put(np.asarray(r, np.int32)), put(np.asarray(c, np.int32)),
+                           put(np.asarray(v, np.float64)))
+    for d in bufs:
+        d.close()
+    return A
+
+
+def test_csr_from_device_memory(ctx):
+    """A matrix created from device arrays gets its longest row and its one-entry-per-row
+    shape measured on the device, so its Galerkin products take the same one-launch paths
+    as a host-created one: the hierarchy and a prolongator-shaped product equal the
+    oracle bit for bit."""
+    oa = O.laplacian3d(27, 12, 12, 12)
+    n = oa.dims()[0]
+    A = _dev_from_device(ctx, oa)
+    assert np.array_equal(A.to_host()[1], oa.arrays()[1])
+    p = params("BCMG")
+    P = bx.prec_setup(ctx, A, p)
+    OP = O.Prec(oa, oparams(p))
+    assert P.levels() == OP.levels() >= 2
+    for l in range(P.levels()):
+        r1, r2 = P.level_matrix(l), OP.level_A(l).to_scipy()
+        assert np.array_equal(r1.indptr, r2.indptr) and np.array_equal(r1.indices, r2.indices)
+        assert np.array_equal(r1.data, r2.data), l
+    w = np.ones(n)
+    agg, pv, nc = O.make_p(O.suitor(oa, w), w)
+    oP = O.Csr.from_arrays(n, nc, np.arange(n + 1), agg, pv)
+    _assert_same_csr(A.spgemm(_dev_from_device(ctx, oP)), O.spgemm(oa, oP))
+    # n entries but not one per row (row 0 empty, row 1 holding two): the general path
+    qcol = np.array(agg, np.int32).copy()
+    qcol[:2] = np.sort(qcol[:2]) if qcol[0] != qcol[1] else [qcol[0], (qcol[0] + 1) % nc]
+    qcol[:2] = np.sort(qcol[:2])
+    oQ = O.Csr.from_arrays(n, nc, np.r_[0, 0, np.arange(2, n + 1)], qcol, pv)
+    _assert_same_csr(A.spgemm(_dev_from_device(ctx, oQ)), O.spgemm(oa, oQ))
+
+
 def test_make_prolongator_exact(ctx):
     for name in ["lap7_9x7x5", "rand_ties", "diff_exp"]:
         oa = CASES[name]()
