@@ -4,9 +4,9 @@
 // GPU.  The reference runs its loop step by step over all rows with separate
 // kernels (d_kapGrad_merge, kapGradCUDA_WiP.cu:189-368; d_gatherFullSysAdapt,
 // d_gFSA.cu:8-79; cudacholdcMix, choldc.cu:24-77; multiscalar_product;
-// check_done, afsai.cu:261-280).  Rows never interact, so here one wave64 owns
-// one row for all steps and keeps its pattern, coefficients and dense system
-// in LDS:
+// check_done, afsai.cu:261-280).  Rows never interact, so here each half of a
+// wave64 owns one row for all steps and keeps its pattern, coefficients and
+// dense system in LDS:
 //   * Kaporin gradient: lane r walks row r of {i} u P (factor 1, then the
 //     current coefficients); columns < i are merged in ascending order with a
 //     wave min, a column's value is summed in lane order; the largest
@@ -50,70 +50,97 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int wave_min_int(int v) {
+// min over the 32 lanes of a half wave
+__device__ __forceinline__ int half_min_int(int v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        int u = __shfl_xor(v, o, 64);
+    for (int o = 16; o > 0; o >>= 1) {
+        int u = __shfl_xor(v, o, 32);
         v = u < v ? u : v;
     }
     return v;
 }
 
+// Two rows per wave, one per half (32 lanes: the row and its at most 30 pattern
+// rows in the gradient): the serial parts (Cholesky diagonals, triangular solves,
+// DKap) run for both rows in one instruction stream.  Cross-lane operations are
+// always executed by the whole wave; a half whose row is done (or past n) only
+// idles through them.  Every floating-point operation of a row is the one of the
+// one-row formulation, in the same order.
 __global__ __launch_bounds__(64 * AF_WAVES) void k_afsai_rows(int n, const int* __restrict__ row, const int* __restrict__ col,
                                                             const double* __restrict__ cs, int* __restrict__ pat_out,
                                                             double* __restrict__ x_out, int* __restrict__ mrow_out,
                                                             double* __restrict__ dk_out) {
-    __shared__ int pat_s[AF_WAVES][AF_MMAX];
-    __shared__ double x_s[AF_WAVES][AF_MMAX];
-    __shared__ double b_s[AF_WAVES][AF_MMAX];
-    __shared__ double L_s[AF_WAVES][AF_PACK];
+    __shared__ int pat_s[2 * AF_WAVES][AF_MMAX];
+    __shared__ double x_s[2 * AF_WAVES][AF_MMAX];
+    __shared__ double b_s[2 * AF_WAVES][AF_MMAX];
+    __shared__ double L_s[2 * AF_WAVES][AF_PACK];
     int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int i = blockIdx.x * AF_WAVES + wid;
-    if (i >= n) return;  // whole wave (LDS is per wave; only wave-level sync below)
-    int* pat = pat_s[wid];
-    double* xs = x_s[wid];
-    double* bb = b_s[wid];
-    double* Lp = L_s[wid];
+    int half = lane >> 5, hl = lane & 31;
+    int slot = 2 * wid + half;
+    long long ri = 2 * ((long long)blockIdx.x * AF_WAVES + wid) + half;
+    bool live = ri < n;
+    if (__ballot(live) == 0) return;  // whole wave (LDS is per half; only wave-level sync below)
+    int i = live ? (int)ri : 0;
+    int* pat = pat_s[slot];
+    double* xs = x_s[slot];
+    double* bb = b_s[slot];
+    double* Lp = L_s[slot];
     int mrow = 0;
     double dk = 0.0, dko = 0.0;
+    bool active = live;  // the half's row still grows (uniform within the half)
     for (int step = 0; step < AF_MMAX; step++) {
+        if (__ballot(active) == 0) break;
         // ---------------- Kaporin gradient over columns < i
         int rr = -1;
         double fac = 0.0;
-        if (lane == 0) { rr = i; fac = 1.0; }
-        else if (lane <= mrow) { rr = pat[lane - 1]; fac = xs[lane - 1]; }
+        if (active) {
+            if (hl == 0) { rr = i; fac = 1.0; }
+            else if (hl <= mrow) { rr = pat[hl - 1]; fac = xs[hl - 1]; }
+        }
         int ptr = 0, end = 0;
         if (rr >= 0) { ptr = row[rr]; end = row[rr + 1]; }
-        int pp = 0;  // pattern cursor (uniform)
+        int pp = 0;  // pattern cursor (uniform within the half)
         int best = -1;
         double bestabs = -1.0, bestval = 0.0;
         while (true) {
             int cc = (rr >= 0 && ptr < end) ? col[ptr] : INT_MAX;
             if (cc >= i) cc = INT_MAX;
-            int jmin = wave_min_int(cc);
-            if (jmin == INT_MAX) break;
-            bool hit = cc == jmin;
+            int jmin = half_min_int(cc);
+            bool more = jmin != INT_MAX;
+            if (__ballot(more) == 0) break;
+            bool hit = more && cc == jmin;
             double contrib = hit ? cs[ptr] * fac : 0.0;
             if (hit) ptr++;
-            unsigned long long m = __ballot(hit);
+            unsigned mh = (unsigned)(__ballot(hit) >> (32 * half));
+            // the column's value: the contributions summed in lane order (both halves
+            // take as many shuffles as the larger count)
+            int cnt = __popc(mh);
+            int c0 = __shfl(cnt, 0, 64), c1 = __shfl(cnt, 32, 64);
+            int cmax = c0 > c1 ? c0 : c1;
             double acc = 0.0;
-            while (m) {
-                int b = __ffsll((long long)m) - 1;
-                acc += __shfl(contrib, b, 64);
-                m &= m - 1;
+            for (int q = 0; q < cmax; q++) {
+                int b = mh ? __ffs(mh) - 1 : 0;
+                double v = __shfl(contrib, b, 32);
+                if (mh) {
+                    acc += v;
+                    mh &= mh - 1;
+                }
             }
-            while (pp < mrow && pat[pp] < jmin) pp++;
-            bool inpat = pp < mrow && pat[pp] == jmin;
-            double a = inpat ? 0.0 : fabs(acc);
-            if (a > bestabs) {
-                bestabs = a;
-                best = jmin;
-                bestval = inpat ? 0.0 : acc;
+            if (more) {
+                while (pp < mrow && pat[pp] < jmin) pp++;
+                bool inpat = pp < mrow && pat[pp] == jmin;
+                double a = inpat ? 0.0 : fabs(acc);
+                if (a > bestabs) {
+                    bestabs = a;
+                    best = jmin;
+                    bestval = inpat ? 0.0 : acc;
+                }
             }
         }
-        if (best < 0 || bestval == 0.0) break;  // no new entry: the row is done
-        // insert `best` into the sorted pattern (lane 0; wave-uniform afterwards)
-        if (lane == 0) {
+        if (active && (best < 0 || bestval == 0.0)) active = false;  // no new entry: the row is done
+        if (__ballot(active) == 0) break;
+        // insert `best` into the sorted pattern (the half's lane 0)
+        if (active && hl == 0) {
             int q = mrow;
             while (q > 0 && pat[q - 1] > best) {
                 pat[q] = pat[q - 1];
@@ -122,13 +149,15 @@ __global__ __launch_bounds__(64 * AF_WAVES) void k_afsai_rows(int n, const int* 
             pat[q] = best;
         }
         wave_sync();
-        mrow++;
+        if (active) mrow++;
         // ---------------- dense system A_s[P,P] and rhs
-        for (int t = lane; t < mrow * (mrow + 1) / 2; t += 64) Lp[t] = 0.0;
-        if (lane < mrow) bb[lane] = 0.0;
+        if (active) {
+            for (int t = hl; t < mrow * (mrow + 1) / 2; t += 32) Lp[t] = 0.0;
+            if (hl < mrow) bb[hl] = 0.0;
+        }
         wave_sync();
-        if (lane < mrow) {
-            int a = lane;
+        if (active && hl < mrow) {
+            int a = hl;
             int r2 = pat[a];
             int jj = row[r2], je = row[r2 + 1];
             for (int b = a; b < mrow; b++) {
@@ -137,7 +166,7 @@ __global__ __launch_bounds__(64 * AF_WAVES) void k_afsai_rows(int n, const int* 
                 if (col[jj] == pat[b]) Lp[b * (b + 1) / 2 + a] = cs[jj];
             }
         }
-        if (lane == 0) {
+        if (active && hl == 0) {
             int jj = row[i], je = row[i + 1], ii = 0;
             while (jj < je && col[jj] < i && ii < mrow) {
                 if (pat[ii] > col[jj]) jj++;
@@ -148,19 +177,25 @@ __global__ __launch_bounds__(64 * AF_WAVES) void k_afsai_rows(int n, const int* 
         wave_sync();
         // ---------------- Cholesky (cudacholdcMix order)
         bool fail = false;
-        for (int ci = 0; ci < mrow; ci++) {
+        int mh0 = __shfl(active ? mrow : 0, 0, 64), mh1 = __shfl(active ? mrow : 0, 32, 64);
+        int mmax = mh0 > mh1 ? mh0 : mh1;
+        for (int ci = 0; ci < mmax; ci++) {
+            bool on = active && !fail && ci < mrow;  // uniform within the half
             // diagonal
             double sum = 0.0;
-            if (lane == 0) {
+            if (on && hl == 0) {
                 sum = Lp[ci * (ci + 1) / 2 + ci];
                 for (int k = 0; k < ci; k++) sum -= Lp[ci * (ci + 1) / 2 + k] * Lp[ci * (ci + 1) / 2 + k];
                 if (!(sum <= 0.0)) Lp[ci * (ci + 1) / 2 + ci] = 1. / sqrt(sum);
             }
-            sum = __shfl(sum, 0, 64);
-            if (sum <= 0.0) { fail = true; break; }
+            sum = __shfl(sum, 0, 32);
+            if (on && sum <= 0.0) {
+                fail = true;
+                on = false;
+            }
             wave_sync();
-            int j = ci + 1 + lane;
-            if (j < mrow) {
+            int j = ci + 1 + hl;
+            if (on && j < mrow) {
                 double s2 = Lp[j * (j + 1) / 2 + ci];
                 for (int k = 0; k < ci; k++) s2 -= Lp[ci * (ci + 1) / 2 + k] * Lp[j * (j + 1) / 2 + k];
                 Lp[j * (j + 1) / 2 + ci] = s2 * Lp[ci * (ci + 1) / 2 + ci];
@@ -168,7 +203,7 @@ __global__ __launch_bounds__(64 * AF_WAVES) void k_afsai_rows(int n, const int* 
             wave_sync();
         }
         // b kept for DKap; x solved in x_s (on breakdown x = b, as the reference leaves it)
-        if (lane == 0) {
+        if (active && hl == 0) {
             for (int t = 0; t < mrow; t++) xs[t] = bb[t];
             if (!fail) {
                 for (int a = 0; a < mrow; a++) {
@@ -187,16 +222,18 @@ __global__ __launch_bounds__(64 * AF_WAVES) void k_afsai_rows(int n, const int* 
             dk = s;
         }
         wave_sync();
-        dk = __shfl(dk, 0, 64);
-        bool stop = (fabs(dk - dko) < AF_EPS * dko) || dk == 0.0;
-        dko = dk;
-        if (stop) break;
+        dk = __shfl(dk, 0, 32);
+        if (active) {
+            bool stop = (fabs(dk - dko) < AF_EPS * dko) || dk == 0.0;
+            dko = dk;
+            if (stop) active = false;
+        }
     }
-    if (lane < mrow) {
-        pat_out[(long long)i * AF_MMAX + lane] = pat[lane];
-        x_out[(long long)i * AF_MMAX + lane] = xs[lane];
+    if (live && hl < mrow) {
+        pat_out[(long long)i * AF_MMAX + hl] = pat[hl];
+        x_out[(long long)i * AF_MMAX + hl] = xs[hl];
     }
-    if (lane == 0) {
+    if (live && hl == 0) {
         mrow_out[i] = mrow;
         dk_out[i] = dk;
     }
@@ -247,7 +284,7 @@ void afsai_build(Context* c, const DCsr& A, DCsr& G, DCsr& GT) {
     BCMGX_LAUNCHED();
     DBuf<int> pat(c, (size_t)std::max(n, 1) * AF_MMAX), mrow(c, std::max(n, 1));
     DBuf<double> xs(c, (size_t)std::max(n, 1) * AF_MMAX), dk(c, std::max(n, 1));
-    k_afsai_rows<<<grid_for(n, AF_WAVES), 64 * AF_WAVES, 0, c->stream>>>(n, A.row.get(), A.col.get(), cs.get(), pat.get(), xs.get(),
+    k_afsai_rows<<<grid_for(n, 2 * AF_WAVES), 64 * AF_WAVES, 0, c->stream>>>(n, A.row.get(), A.col.get(), cs.get(), pat.get(), xs.get(),
                                                                       mrow.get(), dk.get());
     BCMGX_LAUNCHED();
     DBuf<int> len(c, std::max(n, 1));
