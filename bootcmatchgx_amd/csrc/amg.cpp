@@ -518,6 +518,7 @@ struct CycleRun {
     std::vector<LevelWork<V>>* W;
     int k;
     const int* stop;
+    const double* f0 = nullptr;  // level 0's right-hand side in FP64 when its F is not written
 };
 
 template <typename V>
@@ -589,6 +590,10 @@ static V* gamg_cycle(CycleRun<V>& R, int l, const V* F, V* Xin, bool first_sweep
         a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = mat_val<V>(A); a.nnz = A.nnz;
         set_tiles(a, A);
         a.x = cur; a.y = w.Res.get(); a.b = F; a.k = k; a.lanes = lv.lanesA; a.stop = R.stop; a.kchunk = lv.chunkA;
+        if (l == 0 && R.f0) {  // F = (float) r rounded as it is read (the same values)
+            a.b = nullptr;
+            a.b64 = R.f0;
+        }
         spmv<V, V>(c, SpOp::RES, a);
     }
     LevelWork<V>& wc = (*R.W)[l + 1];
@@ -657,22 +662,32 @@ void Prec::refresh_level0() {
 }
 
 bool Prec::fused_entry(int k, FcgEntry* e) {
+    f0_from_r = false;
     if (ptype != BCMGX_PREC_BCMG || p.precision != BCMGX_PRECISION_FP32 || H.levels() < 2 || p.prerelax_sweeps < 1 ||
         n_hrc() > 1)
         return false;
     ensure_work(k);
-    e->F = wf[0].F.get();
+    // level 0's F is read only by the residual after the first sweep when that sweep
+    // is the only pre-smoothing, the first post-smoothing sweep comes from A P and is
+    // the only one, and a coarse correction exists; the residual then takes r (FP64)
+    // when it runs on the pipelined k-column SpMM
+    const Level& l0 = *H.L[0];
+    bool pipelined = k % 4 == 0 && k >= 8 && k <= 64 && l0.chunkA != -1 && (l0.chunkA <= 0 || l0.chunkA >= k);
+    f0_from_r = p.prerelax_sweeps == 1 && p.postrelax_sweeps == 1 && p.post_galerkin && l0.AP.valf &&
+                (H.levels() > 2 || p.relaxnumber_coarse > 0) && pipelined;
+    e->F = f0_from_r ? nullptr : wf[0].F.get();
     e->X = wf[0].X.get();
     e->M = H.L[0]->Mf.get();
     return true;
 }
 
-void Prec::apply_entry_f32(float* z, int k, const int* stop) {
+void Prec::apply_entry_f32(const double* r, float* z, int k, const int* stop) {
     BCMGX_CHECK(ptype == BCMGX_PREC_BCMG && p.precision == BCMGX_PRECISION_FP32 && H.levels() >= 2 && p.prerelax_sweeps >= 1 &&
                     n_hrc() == 1,
                 BCMGX_ERR_STATE, "level-0 entry not applicable");
     ensure_work(k);
     CycleRun<float> R{ctx, this, &H, &wf, k, stop};
+    if (f0_from_r) R.f0 = r;
     float* res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, nullptr, z);
     if (res != z) copy_d2d(ctx, z, res, sizeof(float) * (size_t)A->n * k);
 }
@@ -689,6 +704,7 @@ void Prec::apply_hrc(int h, const double* r, double* u, int k, bool x_zero, cons
         if (entry_ready) {
             BCMGX_CHECK(x_zero && h == 0 && Hh.levels() >= 2 && p.prerelax_sweeps >= 1, BCMGX_ERR_STATE,
                         "level-0 entry not applicable");
+            if (f0_from_r) R.f0 = r;
             res = gamg_cycle<float>(R, 0, wf[0].F.get(), nullptr, true, u);
         } else {
             convert_stop<double, float>(c, r, wf[0].F.get(), nk, stop);
@@ -910,7 +926,7 @@ std::unique_ptr<Prec> prec_setup(Context* c, const DCsr& A, const bcmgx_params& 
 // the widened-FP64 iteration; only the dot products group their sums differently.
 static void fcg_iteration_z32(Context* c, const DCsr& A, Prec* prec, SolveWork& s, int idx) {
     const int* stop = s.flags.get();
-    prec->apply_entry_f32(s.zf.get(), s.k, stop);
+    prec->apply_entry_f32(s.r.get(), s.zf.get(), s.k, stop);
     SpmvArgs a;
     a.n = A.n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
     set_tiles(a, A);
@@ -1024,7 +1040,7 @@ SolveResult fcg_solve(Context* c, const DCsr& A, Prec* prec, const bcmgx_params&
     if (s.z32) {
         // v = B r kept in FP32; w = A v with (r.v, w.v) in the fused product (its third
         // dot, against r here, is not used)
-        prec->apply_entry_f32(s.zf.get(), k, nullptr);
+        prec->apply_entry_f32(s.r.get(), s.zf.get(), k, nullptr);
         SpmvArgs a;
         a.n = n; a.row = A.row.get(); a.col = A.col.get(); a.val = A.val.get(); a.nnz = A.nnz;
         set_tiles(a, A);

@@ -74,16 +74,20 @@ struct Prec {
     void drop_graphs();
     // u <- B r for k interleaved columns; `x_zero`: u holds zeros on entry (the FCG case).
     // `entry_ready`: the level-0 entry of the FP32 cycle (see fused_entry) already
-    // holds F = (float) r and the first sweep X = F / M.
+    // holds F = (float) r (unless the cycle reads r itself) and the first sweep X = F / M.
     void apply(const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready = false);
     // one cycle of bootstrap hierarchy h alone: u <- cycle_h(r) from the guess u
     void apply_hrc(int h, const double* r, double* u, int k, bool x_zero, const int* stop, bool entry_ready = false);
     // the level-0 buffers an FCG update can fill for the next apply (FP32 BCMG
-    // with at least two levels and a pre-smoothing sweep); false otherwise
+    // with at least two levels and a pre-smoothing sweep); false otherwise.  When
+    // the level's right-hand side is read only by the residual after the first
+    // sweep and that runs on the pipelined SpMM, F is left out (e->F = null) and
+    // the cycle rounds r there itself (f0_from_r)
     bool fused_entry(int k, FcgEntry* e);
-    // the FP32 cycle from that entry, its result z left in FP32 (k interleaved
-    // columns): the values the FP64 output would hold, before the exact widening
-    void apply_entry_f32(float* z, int k, const int* stop);
+    bool f0_from_r = false;
+    // the FP32 cycle from that entry for the residual r, its result z left in FP32
+    // (k interleaved columns): the values the FP64 output would hold, before the exact widening
+    void apply_entry_f32(const double* r, float* z, int k, const int* stop);
     // recompute what depends on level 0's values after they changed in place:
     // the l1 row sums M (and Mf) of BCMG, the l1 diagonal of one-level l1-Jacobi
     void refresh_level0();

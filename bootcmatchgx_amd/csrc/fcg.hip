@@ -132,7 +132,7 @@ __global__ __launch_bounds__(WG) void k_norm_entry(int n, int k, int KP, const d
             double rr = r[o];
             acc += rr * rr;
             float fv = (float)rr;
-            e.F[o] = fv;
+            if (e.F) e.F[o] = fv;
             e.X[o] = fv / e.M[i];
         }
     }
@@ -208,9 +208,9 @@ __global__ __launch_bounds__(WG) void k_init_update(int n, int k, const FcgState
     }
     d[t] = vv;                // d = clone(v), q = clone(w)  (FCG.cu:98-99)
     q[t] = ww;
-    if (e.F) {
+    if (e.X) {
         float f = (float)rr;
-        e.F[t] = f;
+        if (e.F) e.F[t] = f;
         e.X[t] = f / e.M[i];
     }
 }
@@ -312,9 +312,9 @@ __global__ __launch_bounds__(WG) void k_update(int n, int k, int KP, const FcgSt
                 if (waiting) x[o] = pend * p1[o] + x[o];
             }
             acc += rr * rr;
-            if (e.F) {
+            if (e.X) {
                 float f = (float)rr;
-                e.F[o] = f;
+                if (e.F) e.F[o] = f;
                 e.X[o] = f / e.M[i];
             }
         }
@@ -376,9 +376,9 @@ __global__ __launch_bounds__(WG) void k_update_z32(int n, int k, int KP, const F
                 if (waiting) x[o] = pend * p[o] + x[o];
             }
             acc += rr * rr;
-            if (e.F) {
+            if (e.X) {
                 float f = (float)rr;
-                e.F[o] = f;
+                if (e.F) e.F[o] = f;
                 e.X[o] = f / e.M[i];
             }
         }

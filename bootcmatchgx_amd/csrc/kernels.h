@@ -56,6 +56,8 @@ struct SpmvArgs {
     const void* x = nullptr;    // V*
     void* y = nullptr;          // V*
     const void* b = nullptr;    // V*  (RES, JACOBI)
+    const double* b64 = nullptr;  // RES with V = float on the pipelined k-column SpMM: b held in FP64,
+                                  // rounded to V as it is read (in place of b)
     const void* D = nullptr;    // V*  (JACOBI)
     double alpha = 1.0, beta = 0.0;
     int k = 1;                  // number of vector columns
@@ -150,6 +152,8 @@ void fcg_norm(Context* c, const FcgBufs& f, const double* v, int mode);
 // Optional level-0 entry of an FP32 V-cycle written by the FCG updates: with
 // the new residual r, F = (float) r and X = F / M (the first l1-Jacobi sweep
 // from a zero guess), so the next preconditioner application starts there.
+// F may be null: then only X is written and the cycle rounds r itself where it
+// needs the level's right-hand side.
 struct FcgEntry {
     float* F = nullptr;
     float* X = nullptr;

@@ -189,6 +189,7 @@ struct SpmmExtra {
     const int* agg = nullptr;      // MODE 4: each row's coarse index (x is the coarse correction e)
     const void* pv = nullptr;      // MODE 4: the prolongator's value of each row (V*)
     const void* xpre = nullptr;    // MODE 4: the pre-smoothed iterate (V*)
+    const double* b64 = nullptr;   // RES (MODE 0): b held in FP64, rounded to V as it is read
 };
 
 // Software-pipelined persistent k-vector SpMM, CPL consecutive columns per
@@ -357,7 +358,8 @@ __global__ __launch_bounds__(WG) void k_spmmp(int n, int k, const int* __restric
             } else {
                 VV* yo = (VV*)(y + o);
                 if (OP == SpOp::AX) *yo = t;
-                else if (OP == SpOp::RES) *yo = *(const VV*)(b + o) - t;
+                else if (OP == SpOp::RES)
+                    *yo = (ex.b64 ? __builtin_convertvector(*(const VD*)(ex.b64 + o), VV) : *(const VV*)(b + o)) - t;
                 else if (OP == SpOp::AXPBY) *yo = t + beta * *yo;
                 else *yo = ((-t + *(const VV*)(b + o)) / D[ri]) + __builtin_convertvector(*(const XV*)(x + o), VV);
             }
@@ -909,7 +911,9 @@ static void launchk(Context* c, const SpmvArgs& a) {
     int kc = a.kchunk > 0 && a.kchunk < a.k ? a.kchunk : a.k;
     // whole rows of x as 16-byte vectors per lane: CPL = 16 / sizeof(V) columns
     constexpr int CPL = 16 / (int)sizeof(V);
-    bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b | (uintptr_t)a.y64) & 15) == 0;
+    bool aligned = (((uintptr_t)x | (uintptr_t)y | (uintptr_t)b | (uintptr_t)a.y64 | (uintptr_t)a.b64) & 15) == 0;
+    if (a.b64 && !(kc == a.k && a.kchunk != -1 && aligned && spmmp_lanes(a.k, CPL) && OP == SpOp::RES))
+        throw Error(BCMGX_ERR_STATE, "spmv: an FP64 right-hand side only on the pipelined k-column residual");
     if (a.kchunk == -1 && aligned && spmmp_lanes(a.k, CPL)) {
         // one launch over all rows (no persistent pipeline): the autotuner's other full-width choice
         int KP = spmmp_lanes(a.k, CPL);
@@ -933,6 +937,7 @@ static void launchk(Context* c, const SpmvArgs& a) {
     }
     if (kc == a.k && aligned && spmmp_lanes(a.k, CPL)) {
         SpmmExtra ex;
+        ex.b64 = a.b64;
         bool nt = a.kchunk == -2;
         if (OP == SpOp::JACOBI && a.y64) {
             ex.y64 = a.y64;
@@ -1004,8 +1009,10 @@ void spmv(Context* c, SpOp op, const SpmvArgs& a) {
     }
     // operands the kernels dereference: a null one is a library bug, reported instead of launched
     if (!a.row || (a.nnz > 0 && (!a.col || !a.val || !a.x)) || !a.y || a.k < 1 ||
-        ((op == SpOp::RES || op == SpOp::JACOBI) && !a.b) || (op == SpOp::JACOBI && (!a.D || !a.x)))
+        (op == SpOp::RES && !a.b && !a.b64) || (op == SpOp::JACOBI && (!a.b || !a.D || !a.x)))
         throw Error(BCMGX_ERR_STATE, "spmv: null operand");
+    if (a.b64 && (a.k == 1 || op != SpOp::RES || !std::is_same<V, float>::value))
+        throw Error(BCMGX_ERR_STATE, "spmv: an FP64 right-hand side only on the pipelined k-column residual");
     if (a.k == 1) {
         int L = a.lanes != 0 ? a.lanes : default_lanes(a.nnz, a.n);  // < 0: CSR-stream kernel
         switch (op) {
