@@ -271,12 +271,17 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
                 diag_index(c, n, Ai->row.get(), Ai->col.get(), dind_i.get());
             }
             suitor_match(c, *Ai, wi.get(), M.get(), Ai == &A0 ? dind0.get() : dind_i.get());
-            int nc = make_prolongator(c, n, M.get(), wi.get(), agg_i.get(), pval_i.get());
+            int npairs = 0;
+            int nc = make_prolongator(c, n, M.get(), wi.get(), agg_i.get(), pval_i.get(), &npairs);
             BCMGX_CHECK(nc > 0, BCMGX_ERR_NUMERIC, "coarsening produced an empty level");
+            // every vertex in a good pair or a singleton of its own: aggregates of at most 2
+            // (a zero-weight singleton joins the aggregate before it, makeP's :41-44)
+            bool pairs_only = (long long)n == (long long)npairs + nc;
             DCsr Pi, Ri, Anew;
             prolongator_csr(c, n, nc, agg_i.get(), pval_i.get(), Pi);
-            transpose_prolongator(c, n, nc, agg_i.get(), pval_i.get(), Ri);
-            galerkin(c, *Ai, Pi, Ri, Anew, 2);  // pairs and singletons: at most 2 entries a row of R
+            if (pairs_only) pair_restriction(c, n, nc, npairs, agg_i.get(), M.get(), pval_i.get(), Ri);
+            else transpose_prolongator(c, n, nc, agg_i.get(), pval_i.get(), Ri);
+            galerkin(c, *Ai, Pi, Ri, Anew, pairs_only ? 2 : -1);
             DBuf<double> wn(c, nc);
             {
                 SpmvArgs a;  // w_{i+1} = R_i w_i, one lane per row (sequential sums)

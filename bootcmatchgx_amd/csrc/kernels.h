@@ -232,8 +232,12 @@ struct MatchWork;
 // dind_checked: A's diagonal positions, already checked to be present in every
 // row (nullptr: found and checked here)
 void suitor_match(Context* c, const DCsr& A, const double* w, int* M, const int* dind_checked = nullptr);
-// makeP_GPU with canonical singleton order; returns the coarse size
-int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval);
+// makeP_GPU with canonical singleton order; returns the coarse size (npairs_out: the good pairs,
+// aggregates 0 .. npairs-1; the coarse size minus it counts the singletons of nonzero weight)
+int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval, int* npairs_out = nullptr);
+// P^T of a pairwise aggregation whose every vertex is in a good pair or a singleton of nonzero
+// weight (n = npairs + coarse size): at most 2 entries a row
+void pair_restriction(Context* c, int n, int nc, int npairs, const int* agg, const int* M, const double* pval, DCsr& R);
 // R = P^T for a one-entry-per-row P given as (agg, pval): rows sorted by fine index
 void transpose_prolongator(Context* c, int n, int nc, const int* agg, const double* pval, DCsr& R);
 // general transpose (CSRm::transpose / Transpose_local)

@@ -595,7 +595,8 @@ __global__ void k_assign_singles(int n, const int* __restrict__ npairs_d, const 
     }
 }
 
-int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval) {
+int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg, double* pval, int* npairs_out) {
+    if (npairs_out) *npairs_out = 0;
     if (n <= 0) return 0;
     DBuf<int> good(c, n), pid(c, (size_t)n + 1), single(c, n), inpair(c, n), sid(c, (size_t)n + 1);
     k_pair_flags<<<grid_for(n, WG), WG, 0, c->stream>>>(n, M, w, good.get(), pval);
@@ -610,7 +611,31 @@ int make_prolongator(Context* c, int n, const int* M, const double* w, int* agg,
     BCMGX_LAUNCHED();
     int npairs, nsingles;
     c->d2h2(&npairs, pid.get() + n, 1, &nsingles, sid.get() + n, 1);
+    if (npairs_out) *npairs_out = npairs;
     return npairs + nsingles;
+}
+
+// The restriction P^T of a pairwise aggregation with no zero-weight singleton
+// (n = 2 npairs + singletons): aggregate c < npairs holds its pair (v < u = M[v]),
+// the others one vertex each, so row c of P^T is [v, u] or [i] -- the rows the
+// general transpose produces (source rows ascending), written in one pass.
+__global__ void k_pair_restriction(int n, int nc, int npairs, const int* __restrict__ agg, const int* __restrict__ M,
+                                   const double* __restrict__ pval, int* __restrict__ rrow, int* __restrict__ rcol,
+                                   double* __restrict__ rval) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= nc) rrow[i] = i < npairs ? 2 * i : npairs + i;
+    if (i >= n) return;
+    int a = agg[i];
+    int pos = a < npairs ? 2 * a + (M[i] < i ? 1 : 0) : npairs + a;
+    rcol[pos] = i;
+    rval[pos] = pval[i];
+}
+void pair_restriction(Context* c, int n, int nc, int npairs, const int* agg, const int* M, const double* pval, DCsr& R) {
+    R.alloc(c, nc, n, n);
+    R.maxrow = npairs > 0 ? 2 : 1;
+    k_pair_restriction<<<grid_for(std::max(n, nc + 1), WG), WG, 0, c->stream>>>(n, nc, npairs, agg, M, pval, R.row.get(),
+                                                                             R.col.get(), R.val.get());
+    BCMGX_LAUNCHED();
 }
 
 // P <- P P_i for one-entry-per-row operators (nsparseMGPU(P, P_i), matchingAggregation.cu:175-191)

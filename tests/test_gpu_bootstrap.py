@@ -155,3 +155,33 @@ def test_bootstrap_stop_rule_and_info(ctx, tmp_path):
     assert f"Estimated ratio                              : {ratio:f}" in txt
     P.finalize()
     A.close()
+
+
+def test_bootstrap_zero_weight_singletons(ctx):
+    """Rows holding only a unit diagonal are smoothed exactly (one l1-Jacobi sweep
+    leaves their error at 0), so the second hierarchy's smooth vector is 0 there:
+    those vertices are singletons of zero weight, which join the aggregate before
+    them (makeP), giving restriction rows of more than two entries.  The hierarchies
+    still equal the oracle's (first bit-exact, second to 1e-10)."""
+    import scipy.sparse as sp
+
+    lap = O.laplacian3d(27, 10, 10, 10).to_scipy()
+    S = sp.block_diag([lap, sp.identity(40)]).tocsr()
+    S.sort_indices()
+    oa = O.Csr.from_scipy(S)
+    A = dev(ctx, oa)
+    p = params("MULTIPLICATIVE", max_hrc=2)
+    P = bx.prec_setup(ctx, A, p)
+    op = O.Prec(oa, oparams(p))
+    assert op.num_hrc()[0] == 2
+    _, pv = op.hrc_level_P(1, 0)
+    assert np.count_nonzero(pv[-40:] == 0.0) == 40  # the zero-weight singletons are there
+    check_hierarchies(P, op)
+    b = np.random.default_rng(13).normal(size=A.n)
+    x, out = bx.solve(ctx, A, b, None, p, P)
+    xo, oi = O.fcg(oa, op, oparams(p), b)
+    assert out["retv"] == 0
+    assert abs(out["niter"] - oi["niter"]) <= 1
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    P.finalize()
+    A.close()
