@@ -250,6 +250,9 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
     else fill<double>(c, w.get(), A0.n, 1.0);
     int ftcoarse = 1;
     double avc = 0.0;
+    // the diagonal positions of the matrix the next sweep matches (written by the
+    // Galerkin product that made it, when its one-launch path ran)
+    DBuf<int> dind_next;
     for (int level = 1; level < p.max_levels;) {
         Level& prev = *H.L[level - 1];
         const DCsr* Ai = &prev.mat();
@@ -265,8 +268,8 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             // A Galerkin product keeps structural entries, so every row of R (A P) holds a
             // diagonal when every row of A does (aggregate c's row gets a_ii of its members):
             // only the fine matrix needs the check
-            DBuf<int> dind_i;
-            if (Ai != &A0) {
+            DBuf<int> dind_i = std::move(dind_next);
+            if (Ai != &A0 && !dind_i) {
                 dind_i.reset(c, std::max(n, 1));
                 diag_index(c, n, Ai->row.get(), Ai->col.get(), dind_i.get());
             }
@@ -277,11 +280,12 @@ static void build_hierarchy(Context* c, const DCsr& A0, const bcmgx_params& p, H
             // every vertex in a good pair or a singleton of its own: aggregates of at most 2
             // (a zero-weight singleton joins the aggregate before it, makeP's :41-44)
             bool pairs_only = (long long)n == (long long)npairs + nc;
-            DCsr Pi, Ri, Anew;
-            prolongator_csr(c, n, nc, agg_i.get(), pval_i.get(), Pi);
+            DCsr Ri, Anew;
             if (pairs_only) pair_restriction(c, n, nc, npairs, agg_i.get(), M.get(), pval_i.get(), Ri);
             else transpose_prolongator(c, n, nc, agg_i.get(), pval_i.get(), Ri);
-            galerkin(c, *Ai, Pi, Ri, Anew, pairs_only ? 2 : -1);
+            DBuf<int> dind_c(c, std::max(nc, 1));
+            if (galerkin(c, *Ai, nc, agg_i.get(), pval_i.get(), Ri, Anew, pairs_only ? 2 : -1, dind_c.get()))
+                dind_next = std::move(dind_c);
             DBuf<double> wn(c, nc);
             {
                 SpmvArgs a;  // w_{i+1} = R_i w_i, one lane per row (sequential sums)

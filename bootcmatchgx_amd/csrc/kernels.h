@@ -246,9 +246,12 @@ void transpose(Context* c, const DCsr& A, DCsr& T);
 void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C);
 // A.maxrow and A.unit_rows from A's row pointers on the device (one read-back)
 void csr_shape(Context* c, DCsr& A);
-// C = R (A P), the same bits as spgemm(R, spgemm(A, P)); A P is not compacted when P is a
-// prolongator.  rmax: a bound on the entries of R's rows (-1: none known)
-void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C, int rmax = -1);
+// C = R (A P) for the prolongator P = (agg, pval) onto nc, the same bits as spgemm(R, spgemm(A, P));
+// A P is not compacted when A's rows fit the one-launch product.  rmax: a bound on the entries of
+// R's rows (-1: none known).  Returns true when it also wrote each row's diagonal position into
+// dind (nc ints, when given)
+bool galerkin(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, const DCsr& R, DCsr& C, int rmax = -1,
+              int* dind = nullptr);
 // A P with FP32 values for the prolongator (agg, pval) onto nc columns, queued with no host
 // synchronisation (C.nnz = -1: the length is at C.row[n]); false when A's rows are too long
 bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, DCsr& C);

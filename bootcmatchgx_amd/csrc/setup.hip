@@ -1203,18 +1203,21 @@ __global__ __launch_bounds__(WG) void k_spgemm_large(int nrows, const int* __res
                   gpre + aoff[r], lscan);
 }
 
+// dind (if given, -1 on entry): the position of each row's diagonal entry
 template <typename O, typename V>
 __global__ void k_spgemm_compact(int n, const O* __restrict__ poff, const int* __restrict__ crow,
                                  const int* __restrict__ tcol, const double* __restrict__ tval, int* __restrict__ ccol,
-                                 V* __restrict__ cval) {
+                                 V* __restrict__ cval, int* __restrict__ dind) {
     int i = (blockIdx.x * WG + (int)threadIdx.x) / 16;
     int lane = (int)threadIdx.x % 16;
     if (i >= n) return;
     long long src = poff[i];
     int dst = crow[i], len = crow[i + 1] - crow[i];
     for (int t = lane; t < len; t += 16) {
-        ccol[dst + t] = tcol[src + t];
+        int cc = tcol[src + t];
+        ccol[dst + t] = cc;
         cval[dst + t] = (V)tval[src + t];
+        if (dind && cc == i) dind[i] = dst + t;
     }
 }
 
@@ -1233,7 +1236,8 @@ __global__ void k_large_sizes(int nl, const int* __restrict__ rows, const long l
 // C's row pointers from the per-row result sizes, its nnz and largest row read back in
 // one round trip, then the rows moved from their product positions (poff) into place
 template <typename O>
-static void spgemm_finish(Context* c, int n, const int* uniq, const O* poff, const int* tcol, const double* tval, DCsr& C) {
+static void spgemm_finish(Context* c, int n, const int* uniq, const O* poff, const int* tcol, const double* tval, DCsr& C,
+                          int* dind = nullptr) {
     C.row.reset(c, (size_t)n + 1);
     DBuf<int> mx(c, 1);
     fill<int>(c, mx.get(), 1, 0);
@@ -1247,8 +1251,9 @@ static void spgemm_finish(Context* c, int n, const int* uniq, const O* poff, con
     C.maxrow = maxrow;
     C.col.reset(c, (size_t)std::max(nnz, 1));
     C.val.reset(c, (size_t)std::max(nnz, 1));
+    if (dind) fill<int>(c, dind, n, -1);
     k_spgemm_compact<O, double><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, poff, C.row.get(), tcol, tval, C.col.get(),
-                                                                           C.val.get());
+                                                                           C.val.get(), dind);
     BCMGX_LAUNCHED();
 }
 
@@ -1325,19 +1330,19 @@ static void spgemm_rows(Context* c, const DCsr& A, const BRows& B, DCsr& C) {
 // 32-lane group: the products of row i sit at A's own entry positions, one
 // launch over all rows with no counting, binning or read-back.  Row i of the
 // result is left at [A.row[i], rend[i]) of tcol / tval.
-static bool unit_product_fits(const DCsr& A, const DCsr& P) {
-    return P.unit_rows && P.nnz == P.n && P.m < (1 << 26) && A.maxrow >= 0 && A.maxrow <= 32;
-}
-static void unit_product_in_place(Context* c, const DCsr& A, const DCsr& P, int* tcol, double* tval, int* uniq, int* rend) {
-    int n = A.n;
+// P given by its aggregates and values (row k of P is the entry (k, agg[k]) = pval[k])
+static bool unit_product_fits(const DCsr& A, int nc) { return nc < (1 << 26) && A.maxrow >= 0 && A.maxrow <= 32; }
+static void unit_product_in_place(Context* c, const DCsr& A, const int* agg, const double* pval, int* tcol, double* tval,
+                                  int* uniq, int* rend) {
+    int n = A.n;  // (one product per entry: B's row pointers are not read)
     if (A.maxrow <= 16)
         k_spgemm_small<16, true, true><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(
-            n, nullptr, A.row.get(), A.col.get(), A.val.get(), P.row.get(), P.row.get() + 1, P.col.get(), P.val.get(), nullptr,
-            tcol, tval, uniq, A.row.get(), rend);
+            n, nullptr, A.row.get(), A.col.get(), A.val.get(), nullptr, nullptr, agg, pval, nullptr, tcol, tval, uniq,
+            A.row.get(), rend);
     else
         k_spgemm_small<32, true, true><<<grid_for(n, WG / 32), WG, 0, c->stream>>>(
-            n, nullptr, A.row.get(), A.col.get(), A.val.get(), P.row.get(), P.row.get() + 1, P.col.get(), P.val.get(), nullptr,
-            tcol, tval, uniq, A.row.get(), rend);
+            n, nullptr, A.row.get(), A.col.get(), A.val.get(), nullptr, nullptr, agg, pval, nullptr, tcol, tval, uniq,
+            A.row.get(), rend);
     BCMGX_LAUNCHED();
 }
 
@@ -1352,11 +1357,11 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
         fill<int>(c, C.row.get(), 1, 0);
         return;
     }
-    if (unit_product_fits(A, B)) {
+    if (B.unit_rows && B.nnz == B.n && unit_product_fits(A, B.m)) {
         size_t tsz = (size_t)std::max(A.nnz, 1);
         DBuf<int> tcol(c, tsz), uniq(c, (size_t)n + 1);
         DBuf<double> tval(c, tsz);
-        unit_product_in_place(c, A, B, tcol.get(), tval.get(), uniq.get(), nullptr);
+        unit_product_in_place(c, A, B.col.get(), B.val.get(), tcol.get(), tval.get(), uniq.get(), nullptr);
         spgemm_finish<int>(c, n, uniq.get(), A.row.get(), tcol.get(), tval.get(), C);
         return;
     }
@@ -1372,13 +1377,11 @@ void spgemm(Context* c, const DCsr& A, const DCsr& B, DCsr& C) {
 // when A does not fit.
 bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, DCsr& C) {
     int n = A.n;
-    if (n <= 0 || nc >= (1 << 26) || A.maxrow < 0 || A.maxrow > 32) return false;
-    DCsr P;
-    prolongator_csr(c, n, nc, agg, pval, P);
+    if (n <= 0 || !unit_product_fits(A, nc)) return false;
     size_t tsz = (size_t)std::max(A.nnz, 1);
     DBuf<int> tcol(c, tsz), uniq(c, (size_t)n + 1);
     DBuf<double> tval(c, tsz);
-    unit_product_in_place(c, A, P, tcol.get(), tval.get(), uniq.get(), nullptr);
+    unit_product_in_place(c, A, agg, pval, tcol.get(), tval.get(), uniq.get(), nullptr);
     C.n = n;
     C.m = nc;
     C.nnz = -1;
@@ -1390,7 +1393,7 @@ bool unit_product_f32_async(Context* c, const DCsr& A, int nc, const int* agg, c
     C.val.release();
     C.valf.reset(c, tsz);
     k_spgemm_compact<int, float><<<grid_for(n, WG / 16), WG, 0, c->stream>>>(n, A.row.get(), C.row.get(), tcol.get(),
-                                                                           tval.get(), C.col.get(), C.valf.get());
+                                                                           tval.get(), C.col.get(), C.valf.get(), nullptr);
     BCMGX_LAUNCHED();
     return true;
 }
@@ -1407,15 +1410,16 @@ __global__ void k_stride_offsets(int n, int stride, int* __restrict__ out) {
 // (the caller's bound: 2 for a pairwise aggregation's restriction) and
 // rmax A.maxrow <= 64, every row of the product fits one 64-lane group: one
 // launch over all rows, 64 product slots per row, no counting or binning.
-void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C, int rmax) {
-    BCMGX_CHECK(A.m == P.n && R.m == A.n, BCMGX_ERR_ARG, "galerkin: dimensions differ");
-    if (A.n > 0 && R.n > 0 && unit_product_fits(A, P)) {
+bool galerkin(Context* c, const DCsr& A, int nc, const int* agg, const double* pval, const DCsr& R, DCsr& C, int rmax,
+              int* dind) {
+    BCMGX_CHECK(R.m == A.n && R.n == nc, BCMGX_ERR_ARG, "galerkin: dimensions differ");
+    if (A.n > 0 && R.n > 0 && unit_product_fits(A, nc)) {
         size_t tsz = (size_t)std::max(A.nnz, 1);
         DBuf<int> tcol(c, tsz), rend(c, (size_t)A.n);
         DBuf<double> tval(c, tsz);
-        unit_product_in_place(c, A, P, tcol.get(), tval.get(), nullptr, rend.get());
+        unit_product_in_place(c, A, agg, pval, tcol.get(), tval.get(), nullptr, rend.get());
         C.n = R.n;
-        C.m = P.m;
+        C.m = nc;
         C.unit_rows = false;
         if (rmax >= 0 && rmax * A.maxrow <= SMALL_P && (long long)R.n * SMALL_P < (1ll << 31)) {
             size_t slots = (size_t)R.n * SMALL_P;
@@ -1427,15 +1431,17 @@ void galerkin(Context* c, const DCsr& A, const DCsr& P, const DCsr& R, DCsr& C, 
                 R.n, nullptr, R.row.get(), R.col.get(), R.val.get(), A.row.get(), rend.get(), tcol.get(), tval.get(), nullptr,
                 ccol.get(), cval.get(), uniq.get(), off.get(), nullptr);
             BCMGX_LAUNCHED();
-            spgemm_finish<int>(c, R.n, uniq.get(), off.get(), ccol.get(), cval.get(), C);
-            return;
+            spgemm_finish<int>(c, R.n, uniq.get(), off.get(), ccol.get(), cval.get(), C, dind);
+            return dind != nullptr;
         }
-        spgemm_rows(c, R, BRows{A.n, P.m, A.row.get(), rend.get(), tcol.get(), tval.get(), false}, C);
-        return;
+        spgemm_rows(c, R, BRows{A.n, nc, A.row.get(), rend.get(), tcol.get(), tval.get(), false}, C);
+        return false;
     }
-    DCsr AP;
+    DCsr P, AP;
+    prolongator_csr(c, A.n, nc, agg, pval, P);
     spgemm(c, A, P, AP);
     spgemm(c, R, AP, C);
+    return false;
 }
 
 // ------------------------------------------------------- row-tile index ---
