@@ -724,8 +724,11 @@ __global__ __launch_bounds__(WG) void k_spmv_tiled(int n, long long nnz, const i
     typedef T T2 __attribute__((ext_vector_type(2)));
     typedef unsigned short U2 __attribute__((ext_vector_type(2)));
     if (stop && *stop) return;
+    // the products overwrite the x entries once every product is formed: one LDS region
+    static_assert(TILE_MAX_COLS <= TILE_MAX_ENTRIES && (TILE_MAX_COLS & (TILE_MAX_COLS - 1)) == 0,
+                  "x fits the products' space; positions mask to it");
     __shared__ V prod[TILE_MAX_ENTRIES];
-    __shared__ V xs[TILE_MAX_COLS];
+    V* xs = prod;
     __shared__ int rp[RB + 1];
     int blk = xcd_order(blockIdx.x, gridDim.x, sb);
     int r0 = blk * RB;
@@ -772,16 +775,25 @@ __global__ __launch_bounds__(WG) void k_spmv_tiled(int n, long long nnz, const i
 #pragma unroll
         for (int p = 0; p < XP; p++) xs[tid + p * WG] = xv[p];
         __syncthreads();
+        V p0[UP], p1[UP];
 #pragma unroll
         for (int q = 0; q < UP; q++) {
-            int t = abase + 2 * (tid + q * WG) - base;  // chunk position of the pair's first entry (may be -1)
             V a0 = (V)a[q].x, a1 = (V)a[q].y;
             if (OP == SpOp::AXPBY) {
                 a0 = alpha * a0;
                 a1 = alpha * a1;
             }
-            if (t >= 0 && t < cnt) prod[t] = a0 * xs[ll[q].x];
-            if (t + 1 >= 0 && t + 1 < cnt) prod[t + 1] = a1 * xs[ll[q].y];
+            // a pair half outside the tile holds a neighbour tile's position, or none (an
+            // unindexed tile's): masked into the x region, its product is never stored
+            p0[q] = a0 * xs[ll[q].x & (TILE_MAX_COLS - 1)];
+            p1[q] = a1 * xs[ll[q].y & (TILE_MAX_COLS - 1)];
+        }
+        __syncthreads();  // every x entry read: the products take the space
+#pragma unroll
+        for (int q = 0; q < UP; q++) {
+            int t = abase + 2 * (tid + q * WG) - base;  // chunk position of the pair's first entry (may be -1)
+            if (t >= 0 && t < cnt) prod[t] = p0[q];
+            if (t + 1 >= 0 && t + 1 < cnt) prod[t + 1] = p1[q];
         }
     }
     if (tid < RB) rp[tid] = myrp;
