@@ -579,8 +579,13 @@ def test_other_cycles_match_oracle(ctx, cycle):
     assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-def test_afsai_factor_vs_oracle(ctx):
-    for oa in [O.laplacian3d(7, 9, 8, 7), O.laplacian3d(27, 7, 7, 7), O.diffusion7p(8, 8, 8, 1, 0.5, 2.0, 4)]:
+def test_afsai_factor_vs_oracle(ctx, golden):
+    """The stencils' rows take the column-table gradient; the Harwell-Boeing matrix and the
+    random one (rows with more than 32 entries left of the diagonal, or more than 192 columns
+    in a row's table) go to the merge kernel, partly or wholly."""
+    cases = [O.laplacian3d(7, 9, 8, 7), O.laplacian3d(27, 7, 7, 7), O.diffusion7p(8, 8, 8, 1, 0.5, 2.0, 4),
+             O.read_mtx(os.path.join(golden, "bcsstk09.mtx")), O.Csr.from_scipy(rand_spd(900, 0.027, 7))]
+    for oa in cases:
         A = dev(ctx, oa)
         p = params("AFSAI")
         G = bx.prec_setup(ctx, A, p).afsai_factor()
