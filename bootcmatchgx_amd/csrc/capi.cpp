@@ -320,6 +320,7 @@ int bcmgx_prec_setup(bcmgx_context ctx, bcmgx_matrix A, const bcmgx_params* p, b
     BCMGX_HIP(hipSetDevice(ctx->c->device));
     BCMGX_CHECK(p->preconditioner >= 0 && p->preconditioner < BCMGX_PREC_INVALID, BCMGX_ERR_PARAM,
                 "Unhandled value for enum class PreconditionerType in prec_setup()");
+    BCMGX_CHECK(A->A.n > 0 && A->A.n == A->A.m, BCMGX_ERR_ARG, "the preconditioner needs a nonempty square matrix");
     auto* h = new bcmgx_prec_s();
     h->ctx = ctx;
     h->A = A;
@@ -478,6 +479,7 @@ int bcmgx_solve(bcmgx_context ctx, bcmgx_matrix A, bcmgx_prec prec, const bcmgx_
     Context* c = ctx->c;
     BCMGX_HIP(hipSetDevice(c->device));
     int n = A->A.n;
+    BCMGX_CHECK(n > 0 && A->A.m == n, BCMGX_ERR_ARG, "the solver needs a nonempty square matrix");
     if (ld < n) ld = n;
     std::unique_ptr<Prec> own;
     Prec* P = prec ? prec->P.get() : nullptr;

@@ -191,7 +191,8 @@ int bcmgx_memcpy(bcmgx_context ctx, void* dst, const void* src, size_t bytes, in
  * (bcmg.cu:65-85), SYMMETRIZED_MULTIPLICATIVE (forward then back, a symmetric
  * preconditioner) or ADDITIVE (corrections summed from one residual).  The
  * reference stops at one hierarchy (bootstrap.cu:37-38) and applies
- * MULTIPLICATIVE only (bcmg.cu:86-89).
+ * MULTIPLICATIVE only (bcmg.cu:86-89).  A matrix with no rows, or not square,
+ * is refused with BCMGX_ERR_ARG before any device work.
  */
 int bcmgx_prec_setup(bcmgx_context ctx, bcmgx_matrix A, const bcmgx_params* p, bcmgx_prec* prec);
 /* prec_apply (src/preconditioner/prec_apply.h:32): u <- B r.  As in the
@@ -242,7 +243,8 @@ int bcmgx_prec_afsai_factor(bcmgx_prec prec, int* n, int* nnz, int* row, int* co
  * scalars and stopping test; the s-step forms solve them one after another.
  * Stopping: FCG ||r_k|| <= rtol ||r_0|| (FCG.cu:93,156); the others
  * ||r_k|| < rtol delta0 with delta0 = ||r_0|| for stop_criterion 1, else 1
- * (CG_HS.cu:74-79,141).
+ * (CG_HS.cu:74-79,141).  A matrix with no rows, or not square, is refused
+ * with BCMGX_ERR_ARG.
  */
 typedef struct bcmgx_solver_out {
     int retv;               /* 0 converged, -1 itnlim reached (FCG.cu:160-162), > 0 the s-step

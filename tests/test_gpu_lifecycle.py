@@ -169,3 +169,27 @@ def test_update_values_refreshes_level0_divisors(ctx, precision, tol):
     J.finalize()
     A.close()
     B2.close()
+
+
+@pytest.mark.parametrize("prec", ["NONE", "L1_JACOBI", "BCMG", "AFSAI"])
+def test_empty_or_rectangular_system_is_refused(prec):
+    """A matrix with no rows, or a rectangular one, gets BCMGX_ERR_ARG from setup and
+    from solve before any device work; the context stays usable."""
+    c = bx.Context(0)
+    empty = bx.CSR(c, 0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    rect = bx.CSR(c, 3, 4, np.arange(4, dtype=np.int32), np.array([0, 1, 2], np.int32), np.ones(3))
+    for M in (empty, rect):
+        with pytest.raises(bx.BcmgxError) as e:
+            bx.prec_setup(c, M, params(prec))
+        assert e.value.status == abi.ERR_ARG, e.value
+        B, X = bx.DeviceArray(c, (4, 1)), bx.DeviceArray(c, (4, 1))
+        with pytest.raises(bx.BcmgxError) as e:
+            bx.solve_device(c, M, B.ptr, X.ptr, 1, params(prec), ld=4)
+        assert e.value.status == abi.ERR_ARG, e.value
+        B.close()
+        X.close()
+    oa = O.laplacian3d(7, 6, 6, 6)
+    A = dev(c, oa)
+    x, out = bx.solve(c, A, np.ones(A.n), None, params(prec), bx.prec_setup(c, A, params(prec)))
+    assert out["retv"] == 0
+    c.close()
