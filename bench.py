@@ -302,6 +302,8 @@ def main():
                     help="only time the smoother and the Galerkin SpGEMMs of the fine matrix (profiling)")
     ap.add_argument("--solve-only", action="store_true",
                     help="only one setup and one batched solve of the headline workload (profiling)")
+    ap.add_argument("--afsai-only", type=int, default=0, metavar="REPS",
+                    help="only REPS AFSAI setups of the 27-point matrix at --size (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -333,6 +335,17 @@ def main():
         res = kernel_rooflines(bx, ctx, A, n, nnz, k, args.kernels_only)
         res["copy_gbs"] = ctx.stream_copy_gbs(1 << 30, 20)  # k_copy16 over 1 GiB: counter calibration
         print(json.dumps(res))
+        return
+    if args.afsai_only:
+        p = bx.Params(solver="FCG", preconditioner="AFSAI", rtol=rtol, dispnorm=0)
+        times = []
+        for _ in range(args.afsai_only):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            P = bx.prec_setup(ctx, A, p)
+            times.append(time.perf_counter() - t0)
+            P.finalize()
+        print(json.dumps({"afsai_setup_s": times, "n": n, "nnz": nnz}))
         return
     if args.solve_only:
         B = bx.DeviceArray(ctx, (n, k))

@@ -687,11 +687,43 @@ __global__ void k_scatter_T(int n, const int* __restrict__ row, const int* __res
         tval[p] = val[j];
     }
 }
-__global__ void k_sort_rows(int n, const int* __restrict__ row, int* __restrict__ col, double* __restrict__ val) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int s = row[i], e = row[i + 1];
-    for (int a = s + 1; a < e; a++) {  // insertion sort (rows are short)
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m, int w) {
+    return (unsigned long long)__shfl_xor((long long)v, m, w);
+}
+
+// Each row sorted by column in place, equal columns keeping their order; one wave a row.
+// Rows of at most 64 entries: a bitonic sort of the (column, position) keys across the
+// lanes, each value then fetched from the lane it came from.  Longer rows: one lane's
+// insertion sort.
+__global__ __launch_bounds__(WG) void k_sort_rows(int n, const int* __restrict__ row, int* __restrict__ col,
+                                                  double* __restrict__ val) {
+    int lane = threadIdx.x & 63;
+    long long i = ((long long)blockIdx.x * WG + threadIdx.x) >> 6;
+    if (i >= n) return;  // whole wave
+    int s = row[i], e = row[i + 1], len = e - s;
+    if (len <= 1) return;
+    if (len <= 64) {
+        unsigned long long key = lane < len ? ((unsigned long long)(unsigned)col[s + lane] << 32) | (unsigned)lane : ~0ull;
+        double v = lane < len ? val[s + lane] : 0.0;
+#pragma unroll
+        for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                unsigned long long pk = shfl_xor_u64(key, j, 64);
+                bool up = (lane & kk) == 0;
+                bool lower = (lane & j) == 0;
+                if ((lower == up) ? (pk < key) : (pk > key)) key = pk;
+            }
+        }
+        double sv = __shfl(v, (int)(key & 63u), 64);
+        if (lane < len) {
+            col[s + lane] = (int)(key >> 32);
+            val[s + lane] = sv;
+        }
+        return;
+    }
+    if (lane != 0) return;
+    for (int a = s + 1; a < e; a++) {
         int kc = col[a];
         double kv = val[a];
         int b = a - 1;
@@ -719,7 +751,7 @@ static void transpose_raw(Context* c, int n, int m, long long nnz, const int* ro
         BCMGX_LAUNCHED();
     }
     if (m > 0) {
-        k_sort_rows<<<grid_for(m, 64), 64, 0, c->stream>>>(m, T.row.get(), T.col.get(), T.val.get());
+        k_sort_rows<<<grid_for(m, WG / 64), WG, 0, c->stream>>>(m, T.row.get(), T.col.get(), T.val.get());
         BCMGX_LAUNCHED();
     }
 }
@@ -901,10 +933,6 @@ __global__ __launch_bounds__(WG) void k_bin_rows(int n, const int* __restrict__ 
 #pragma unroll
     for (int q = 0; q < BIN_CHUNKS; q++)
         if (bq[q] >= 0) bins[(long long)bq[q] * cap + lbase[bq[q]] + slot[q]] = (int)(i0 + (long long)q * WG);
-}
-
-__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m, int w) {
-    return (unsigned long long)__shfl_xor((long long)v, m, w);
 }
 
 // W lanes per row (W = 16, 32 or 64), products <= W: 64 / W rows per wave.

@@ -314,6 +314,25 @@ def test_spgemm_galerkin_bit_exact(ctx, name):
     _assert_same_csr(Pd.transpose(), R)
 
 
+@pytest.mark.parametrize("name", ["lap27_10", "long_rows", "rect_empty_rows", "mixed_lengths"])
+def test_transpose_equals_scipy(ctx, name):
+    """Transposed rows sorted by column: rows of at most 64 entries by the one-wave bitonic
+    sort, longer ones (120 and more) by one lane; empty rows and rectangular shapes."""
+    S = {"lap27_10": lambda: CASES["lap27_10"]().to_scipy(),
+         "long_rows": lambda: rand_spd(300, 0.4, 7),
+         "rect_empty_rows": lambda: sp.random(260, 190, density=0.05, random_state=8, format="csr"),
+         "mixed_lengths": lambda: sp.hstack([sp.random(400, 30, density=0.5, random_state=13),
+                                             sp.random(400, 300, density=0.02, random_state=14)]).tocsr()}[name]()
+    S = sp.csr_matrix(S)
+    S.sort_indices()
+    T = bx.CSR.from_scipy(ctx, S).transpose()
+    r, c, v = T.to_host()
+    Ts = S.T.tocsr()
+    Ts.sort_indices()
+    assert np.array_equal(r, Ts.indptr) and np.array_equal(c, Ts.indices)
+    assert np.array_equal(v, Ts.data)
+
+
 def test_spgemm_medium_and_large_rows(ctx):
     # A*A of a 27-point stencil: up to 729 products per row (LDS bin)
     oa = O.laplacian3d(27, 9, 8, 7)
