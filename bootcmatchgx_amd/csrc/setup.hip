@@ -964,10 +964,19 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     bool active = w < nrows;  // inactive groups still take part in the barriers
     int i = active ? (rows ? rows[w] : w) : 0;
     int s = active ? arow[i] : 0, alen = active ? arow[i + 1] - s : 0;
-    int k = -1, cntl = 0;
+    // lane a holds A's entry a (at most W of them: rows are binned by max(products, entries)):
+    // its column, and for a general B its value and B row start, which the product lanes then
+    // take by shuffle instead of loading them again (with unit rows each lane's product is its
+    // own entry, read directly)
+    int k = -1, cntl = 0, kb = 0;
+    double ka_v = 0.0;
     if (lane < alen) {
         k = acol[s + lane];
-        cntl = UNITB ? 1 : bend[k] - brow[k];
+        if constexpr (!UNITB) {
+            ka_v = aval[s + lane];
+            kb = brow[k];
+        }
+        cntl = UNITB ? 1 : bend[k] - kb;
     }
     // inclusive scan of B-row lengths across the group (with unit rows: lane + 1, up to alen)
     int x = cntl;
@@ -996,8 +1005,10 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     unsigned long long key = ~0ull;
     unsigned key32 = ~0u;
     double v = 0.0;
+    // owner: largest a with pre[a] <= pos (with unit rows: pos itself); the shuffles run on
+    // every lane, outside the branch, so each source lane is active
+    int a = 0;
     if (pos < total) {
-        // owner: largest a with pre[a] <= pos (with unit rows: pos itself)
         int lo = 0, hi = alen;  // pre[0] = 0 <= pos
         if constexpr (UNITB) {
             lo = pos;
@@ -1008,11 +1019,23 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
                 else hi = mid;
             }
         }
-        int a = lo;
-        int ka = acol[s + a];
-        int jb = UNITB ? ka : brow[ka] + (pos - pre_s[gid][a]);
+        a = lo;
+    }
+    int ka = 0, kba = 0;
+    double ava = 0.0;
+    if constexpr (!UNITB) {
+        ka = __shfl(k, a, W);
+        kba = __shfl(kb, a, W);
+        ava = __shfl(ka_v, a, W);
+    }
+    if (pos < total) {
+        if constexpr (UNITB) {
+            ka = acol[s + a];
+            ava = aval[s + a];
+        }
+        int jb = UNITB ? ka : kba + (pos - pre_s[gid][a]);
         int cc = bcol[jb];
-        v = aval[s + a] * bval[jb];
+        v = ava * bval[jb];
         key = ((unsigned long long)(unsigned)cc << 32) | (unsigned)pos;
         key32 = ((unsigned)cc << 6) | (unsigned)pos;
     }
