@@ -963,6 +963,8 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
     int w = blockIdx.x * G + gid;
     bool active = w < nrows;  // inactive groups still take part in the barriers
     int i = active ? (rows ? rows[w] : w) : 0;
+    // one row per wave (W = 64): the row, and so its bounds, are the wave's own (scalar loads)
+    if constexpr (W == 64) i = __builtin_amdgcn_readfirstlane(i);
     int s = active ? arow[i] : 0, alen = active ? arow[i + 1] - s : 0;
     // lane a holds A's entry a (at most W of them: rows are binned by max(products, entries)):
     // its column, and for a general B its value and B row start, which the product lanes then
