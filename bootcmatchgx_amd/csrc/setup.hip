@@ -1115,14 +1115,15 @@ __global__ __launch_bounds__(WG) void k_spgemm_small(int nrows, const int* __res
             else break;
         }
         int u = __popcll(hm & ((1ull << lane) - 1ull));
-        long long o = (poff32 ? (long long)poff32[i] : poff[i]) + u;
+        // an in-place product writes at A's own row start, which this group holds
+        long long o = (poff32 ? (poff32 == arow ? (long long)s : (long long)poff32[i]) : poff[i]) + u;
         tcol[o] = colv;
         tval[o] = acc;
     }
     if (lane == 0 && active) {
         int u = __popcll(hm);
         if (uniq) uniq[i] = u;
-        if (rend) rend[i] = poff32[i] + u;
+        if (rend) rend[i] = (poff32 == arow ? s : poff32[i]) + u;
     }
 }
 
